@@ -1,0 +1,247 @@
+/*
+ * tfm.h - C ABI of libtfm_hip.so, the MI355X (gfx950) assemble-and-solve path
+ * behind the torchfem Python API.
+ *
+ * Conventions (the same ones the reference uses for its only native binding,
+ * the AmgX ctypes table in src/torchfem/amgx.py:100-201):
+ *   - every entry point returns an int status, TFM_OK (0) on success;
+ *     tfm_last_error() returns a message for the last failure of this thread;
+ *   - opaque handles come from *_create(&h, ...) and are released by an
+ *     explicit *_destroy(h) (amgx.py:204-208, 385-395);
+ *   - device arrays are raw pointers owned by the caller (torch tensors on the
+ *     caller side); the library allocates only inside explicit handles;
+ *   - every device call takes the caller's hipStream_t and is stream ordered;
+ *     the only host synchronisations are the convergence checks of the Krylov
+ *     solver (the reference syncs there too, sparse.py:579-596).
+ *
+ * Index conventions follow the reference: local DOF order inside an element is
+ * node-major / component-minor (base.py:73-76) and the global matrix pattern is
+ * the sorted set of (row, col) pairs of all element couplings plus the full
+ * diagonal, stored row-major (base.py:79-118).
+ */
+#ifndef TFM_H
+#define TFM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *tfm_stream_t; /* a hipStream_t; NULL is the null stream */
+
+enum {
+  TFM_OK = 0,
+  TFM_ERR_ARG = 1,     /* invalid argument (sizes, pointers, unsupported type) */
+  TFM_ERR_HIP = 2,     /* a HIP runtime call failed */
+  TFM_ERR_NOCONV = 3,  /* the Krylov solver did not reach its tolerance */
+  TFM_ERR_JACOBIAN = 4 /* an element has a non-positive Jacobian determinant */
+};
+
+/* Element types (src/torchfem/elements.py). */
+enum {
+  TFM_QUAD1 = 1,  /* 4-node bilinear quad, 2x2 Gauss (elements.py:491-567)   */
+  TFM_TETRA1 = 2, /* 4-node linear tetrahedron, 1 point (elements.py:727-790) */
+  TFM_HEXA1 = 3,  /* 8-node trilinear hexahedron, 2x2x2 (elements.py:936-1075) */
+  TFM_HEXA2 = 4   /* 20-node serendipity hexahedron, 2x2x2 (elements.py:1078-1406) */
+};
+
+/* Material parameter layouts. */
+enum {
+  /* Isotropic elasticity: C = lam I(x)I + G (dik djl + dil djk) with per-element
+   * lam (p0) and G (p1). Plane stress passes lam' = E nu / (1 - nu^2)
+   * (materials/elasticity.py:44-70, 164-182). */
+  TFM_MAT_ISO = 1,
+  /* General linear elastic tangent C[e, i, j, k, l] (p0), dim^4 per element. */
+  TFM_MAT_GENERAL = 2,
+  /* Isotropic conductivity KAPPA = kappa I with per-element kappa (p0)
+   * (materials/conductivity.py:30-45). */
+  TFM_MAT_HEAT_ISO = 3,
+  /* General conductivity KAPPA[e, i, j] (p0), dim^2 per element. */
+  TFM_MAT_HEAT_GENERAL = 4
+};
+
+/* Solver precision modes. */
+enum {
+  TFM_PREC_FP64 = 0,  /* all-FP64 PCG on FP64 matrix values                     */
+  TFM_PREC_MIXED = 1  /* FP32 inner PCG on FP32 values + FP64 iterative refinement
+                         on the matrix-free FP64 residual                        */
+};
+
+/* Preconditioners. */
+enum {
+  TFM_PC_JACOBI = 0,      /* inverse of diag(K) (cupy path, sparse.py:406-421) */
+  TFM_PC_BLOCK_JACOBI = 1 /* inverse of the ndof x ndof nodal diagonal blocks  */
+};
+
+/* Mesh and node-block sparsity pattern, all arrays on the device.
+ * The CSR values of node n's ndof rows are contiguous: entry (a, p, b) of
+ * node n (row n*ndof+a, column adj[adj_ptr[n]+p]*ndof+b) sits at
+ * val_ptr[n] + a*ndof*deg(n) + p*ndof + b, which is exactly the row-major
+ * order of the reference's glob_idx (base.py:91, 112-117). A node that belongs
+ * to no element ("orphan", deg == 0) holds only its ndof diagonal entries. */
+typedef struct tfm_mesh {
+  const double *nodes;       /* [n_nod, dim] */
+  const int32_t *elements;   /* [n_elem, npe] */
+  int64_t n_nod;
+  int64_t n_elem;
+  int32_t dim;               /* spatial dimension (2 or 3) */
+  int32_t npe;               /* nodes per element */
+  int32_t ndof;              /* DOFs per node (dim for mechanics, 1 for heat) */
+  int32_t etype;             /* TFM_QUAD1 ... TFM_HEXA2 */
+  int32_t max_deg;           /* max node degree (adjacent nodes incl. self) */
+  int32_t max_inc;           /* max number of elements incident to a node */
+  int64_t nnz;               /* number of stored matrix entries */
+  const int32_t *adj_ptr;    /* [n_nod + 1] */
+  const int32_t *adj;        /* [adj_ptr[n_nod]] sorted adjacent nodes, incl. self */
+  const int64_t *val_ptr;    /* [n_nod + 1] first CSR value of the node's rows */
+  const int32_t *ne_ptr;     /* [n_nod + 1] node -> element incidence */
+  const int32_t *ne_elem;    /* [n_elem*npe] incident elements, ascending */
+  const int32_t *ne_loc;     /* [n_elem*npe] local index of the node there */
+  const uint16_t *epos;      /* [n_elem, npe, npe] position of node j in adj(node i) */
+} tfm_mesh_t;
+
+/* Per-element material parameters (device). */
+typedef struct tfm_material {
+  int32_t kind;              /* TFM_MAT_* */
+  int32_t _pad;
+  const double *p0;          /* see TFM_MAT_* */
+  const double *p1;          /* see TFM_MAT_* */
+  const double *scale;       /* [n_elem] volume scale (planar thickness) or NULL */
+} tfm_material_t;
+
+/* An assembled (or matrix-free) linear system K u = b with Dirichlet DOFs. */
+typedef struct tfm_system {
+  const tfm_mesh_t *mesh;
+  const tfm_material_t *mat;
+  const uint8_t *con;        /* [n_nod*ndof] 1 where the DOF is constrained */
+  const double *val64;       /* FP64 CSR values or NULL */
+  const float *val32;        /* FP32 CSR values or NULL */
+  const double *diag_blocks; /* [n_nod, ndof, ndof] nodal diagonal blocks of K */
+} tfm_system_t;
+
+typedef struct tfm_pattern tfm_pattern_t;
+typedef struct tfm_pcg tfm_pcg_t;
+
+/* ---------------------------------------------------------------- errors */
+const char *tfm_last_error(void);
+int tfm_version(void);
+
+/* ---------------------------------------------- sparsity pattern (host) */
+/* Replaces FEM.__init__'s pattern block (base.py:72-122): builds the node
+ * adjacency, the node->element incidence and the element -> slot positions
+ * from a host connectivity array. Integer work only, on the host. */
+int tfm_pattern_create(tfm_pattern_t **h, const int64_t *elements, int64_t n_elem,
+                       int32_t npe, int64_t n_nod, int32_t ndof);
+/* Sizes: nnz, total adjacency length, max degree, max incidence. */
+int tfm_pattern_info(const tfm_pattern_t *h, int64_t *nnz, int64_t *n_adj,
+                     int32_t *max_deg, int32_t *max_inc);
+/* Copies the arrays into caller host buffers sized from tfm_pattern_info. */
+int tfm_pattern_export(const tfm_pattern_t *h, int32_t *adj_ptr, int32_t *adj,
+                       int64_t *val_ptr, int32_t *ne_ptr, int32_t *ne_elem,
+                       int32_t *ne_loc, uint16_t *epos);
+int tfm_pattern_destroy(tfm_pattern_t *h);
+
+/* ------------------------------------------- pattern expansion (device) */
+/* The reference's index arrays on the device: glob_idx rows/cols (int64,
+ * base.py:112-117), CSR row pointer, k_map (int32 [n_elem*(npe*ndof)^2],
+ * base.py:94-104) and diag_map (int32 [n_dofs], base.py:106-108).
+ * Any output pointer may be NULL. */
+int tfm_pattern_expand(tfm_stream_t s, const tfm_mesh_t *m, int64_t *rows,
+                       int64_t *cols, int64_t *rowptr, int32_t *k_map,
+                       int32_t *diag_map);
+
+/* --------------------------------------------- element integration (device) */
+/* Element matrices k[e] = sum_gp w detJ B^T C B (times scale), FP64
+ * [n_elem, npe*ndof, npe*ndof]; replaces the BCB einsum of
+ * Mechanics/Heat.integrate_material (base.py:1086-1090, 1270-1276) and
+ * FEM.k0 (base.py:228-245). *bad_jac is set to 1 if some detJ <= 0. */
+int tfm_element_matrices(tfm_stream_t s, const tfm_mesh_t *m,
+                         const tfm_material_t *mat, double *k, int32_t *bad_jac);
+
+/* Fused integration + deterministic assembly + Dirichlet epilogue: writes the
+ * CSR values of K in FP64 and/or FP32 (either may be NULL) and the nodal
+ * diagonal blocks. Replaces integrate_material's k plus
+ * FEM.assemble_matrix (base.py:398-426) with no float atomics. */
+int tfm_assemble(tfm_stream_t s, const tfm_mesh_t *m, const tfm_material_t *mat,
+                 const uint8_t *con, double *val64, float *val32,
+                 double *diag_blocks, int32_t *bad_jac);
+
+/* Same Dirichlet epilogue and outputs as tfm_assemble, from caller-supplied
+ * element matrices k [n_elem, npe*ndof, npe*ndof] (FEM.assemble_matrix,
+ * base.py:398-426, as an ordered gather instead of index_add_). */
+int tfm_assemble_elements(tfm_stream_t s, const tfm_mesh_t *m, const double *k,
+                          const uint8_t *con, double *val64, float *val32,
+                          double *diag_blocks);
+
+/* Internal forces and fluxes for a nodal field u (FP64):
+ * element forces fe[e][j][a] = sum_gp w detJ B^T (flux_prev + flux(grad u))
+ * written to work [n_elem, npe, ndof]; f[n_nod*ndof] (may be NULL) gathers them
+ * in ascending element order (FEM.assemble_rhs, base.py:428-445). Gradients
+ * and fluxes per Gauss point ([n_gp, n_elem, ndof, dim]) and their Gauss-point
+ * means ([n_elem, ndof, dim]) are written where the pointer is not NULL.
+ * flux_prev [n_gp, n_elem, ndof, dim] may be NULL (zero). Replaces the force
+ * part of integrate_material (base.py:1050-1083, 1250-1268). */
+int tfm_internal_force(tfm_stream_t s, const tfm_mesh_t *m,
+                       const tfm_material_t *mat, const double *u,
+                       const double *flux_prev, double *f, double *grad_mean,
+                       double *flux_mean, double *grad_gp, double *flux_gp,
+                       double *work);
+
+/* F[n*ndof+a] = sum over the elements of node n (ascending) of
+ * fe[e][local][a]: FEM.assemble_rhs (base.py:428-445) without float atomics. */
+int tfm_gather_nodes(tfm_stream_t s, const tfm_mesh_t *m, const double *fe, double *F);
+
+/* Per-element parameter sensitivities of lam^T K(p) u for nrhs pairs of nodal
+ * fields (u, lam stored [nrhs, n_dofs]); out is [nrhs, n_elem, n_out] with
+ *   TFM_MAT_ISO:          n_out = 3: d/dlam, d/dG, d/dscale
+ *   TFM_MAT_GENERAL:      n_out = dim^4 + 1: d/dC[i,j,k,l], d/dscale
+ *   TFM_MAT_HEAT_ISO:     n_out = 2: d/dkappa, d/dscale
+ *   TFM_MAT_HEAT_GENERAL: n_out = dim^2 + 1: d/dKAPPA[i,j], d/dscale
+ * scaled by `sign`. With sign = -1, lam the adjoint and u the solution this is
+ * the implicit-function gradient -lam^T dR/dp of NewtonRaphsonAdjoint.backward
+ * (sparse.py:641-724) for a linear material, with K_e recomputed, never stored. */
+int tfm_sensitivity(tfm_stream_t s, const tfm_mesh_t *m, const tfm_material_t *mat,
+                    const double *u, const double *lam, int32_t nrhs, double sign,
+                    double *out);
+
+/* ----------------------------------------------------- sparse kernels */
+/* y = K x on the node-block CSR values, FP64 (prec 0) or FP32 (prec 1). */
+int tfm_spmv(tfm_stream_t s, const tfm_mesh_t *m, int32_t prec, const void *vals,
+             const void *x, void *y);
+
+/* y = A x for a general CSR matrix (int64 row pointer, int32 columns), FP64;
+ * the operator behind torchfem.sparse on matrices without a mesh. */
+int tfm_csr_spmv(tfm_stream_t s, int64_t n, const int64_t *rowptr,
+                 const int32_t *col, const double *val, const double *x, double *y);
+
+/* ------------------------------------------------------------ PCG solver */
+/* Workspace handle for up to nrhs simultaneous right-hand sides of size n
+ * (like the reference's AmgXSolver handle, amgx.py:211-395). */
+int tfm_pcg_create(tfm_pcg_t **h, int64_t n, int32_t nrhs, int32_t max_iter);
+int tfm_pcg_destroy(tfm_pcg_t *h);
+
+/* Solves K x = b for nrhs right-hand sides (b, x stored [nrhs, n], FP64; x holds
+ * the initial guess on entry, cf. the warm start of sparse.py:103-128).
+ * prec: TFM_PREC_FP64 or TFM_PREC_MIXED; pc: TFM_PC_*. Stops when
+ * ||b - K x||_2 <= rtol ||b||_2 for every right-hand side (the criterion of
+ * the reference's CG/MINRES with rtol = stol, sparse.py:406-421).
+ * iters[nrhs] receives the total inner iterations, relres[nrhs] the final FP64
+ * relative residual, outer[0] the refinement steps (mixed mode).
+ * Returns TFM_ERR_NOCONV when max_iter is exhausted. */
+int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t s, const tfm_system_t *sys,
+                  int32_t prec, int32_t pc, const double *b, double *x,
+                  int32_t nrhs, double rtol, double inner_rtol, int32_t *iters,
+                  double *relres, int32_t *outer);
+
+/* Preconditioned CG for a general SPD CSR matrix (FP64, Jacobi). */
+int tfm_csr_pcg_solve(tfm_pcg_t *h, tfm_stream_t s, int64_t n,
+                      const int64_t *rowptr, const int32_t *col,
+                      const double *val, const double *b, double *x, double rtol,
+                      int32_t *iters, double *relres);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TFM_H */

@@ -1,0 +1,206 @@
+"""GPU parity of the individual HIP kernels against the CPU oracle.
+
+Integer outputs (pattern indices) must match bit for bit; FP64 kernels to
+1e-12 relative (only the summation order differs from the oracle's einsum /
+index_add_); FP32 matrix values must be the exact rounding of the FP64 ones.
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+import oracle as O
+from torchfem import Planar, Solid, SolidHeat
+from torchfem import _lib as L
+from torchfem.base import _MatHolder, _System
+from torchfem.materials import (IsotropicConductivity3D, IsotropicElasticity3D,
+                                IsotropicElasticityPlaneStress)
+
+pytestmark = pytest.mark.gpu
+DT = torch.float64
+
+
+def _case(name, general=False):
+    """(product model, oracle model) pairs on small meshes of every element type."""
+    g = torch.Generator().manual_seed(1)
+    if name == "Quad1":
+        nodes, el = O.rect_quad(5, 4, 2.0, 1.0)
+        E = 800.0 + 400.0 * torch.rand(len(el), generator=g, dtype=DT)
+        t = 0.5 + torch.rand(len(el), generator=g, dtype=DT)
+        mat = IsotropicElasticityPlaneStress(E, 0.3)
+        m = Planar(nodes.cuda(), el.cuda(), mat, thickness=t.cuda())
+        o = O.OracleModel(nodes, el, O.Quad1, "planar", O.plane_stress_C(E, torch.tensor(0.3)), thickness=t)
+        return m, o
+    if name == "Tetra1-heat":
+        nodes, el = O.cube_tetra6(3, jitter=0.1, seed=0)
+        k = torch.exp(0.5 * torch.randn(len(el), generator=g, dtype=DT))
+        m = SolidHeat(nodes.cuda(), el.cuda(), IsotropicConductivity3D(k))
+        o = O.OracleModel(nodes, el, O.Tetra1, "heat", O.iso_KAPPA(k))
+        return m, o
+    if name == "Tetra1":
+        nodes, el = O.cube_tetra6(2, jitter=0.1, seed=3)
+    elif name == "Hexa1":
+        nodes, el = O.cube_hexa(4, 3, 5)
+        nodes = nodes + 0.05 * torch.sin(3.0 * nodes[:, [1, 2, 0]])  # distorted elements
+    else:
+        nodes, el = O.linear_to_quadratic(*O.cube_hexa(3, 3, 4))
+        nodes = nodes + 0.03 * torch.sin(3.0 * nodes[:, [1, 2, 0]])
+    E = 800.0 + 400.0 * torch.rand(len(el), generator=g, dtype=DT)
+    mat = IsotropicElasticity3D(E, 0.3)
+    Cfull = O.iso3d_C(E, torch.tensor(0.3))
+    if general:
+        # an anisotropic tangent exercises the general layout
+        Cfull = Cfull.clone()
+        Cfull[:, 0, 0, 0, 0] *= 1.7
+        Cfull[:, 1, 2, 1, 2] *= 0.6
+        Cfull[:, 2, 1, 2, 1] *= 0.6
+        Cfull[:, 1, 2, 2, 1] *= 0.6
+        Cfull[:, 2, 1, 1, 2] *= 0.6
+        mat = IsotropicElasticity3D(E, 0.3).vectorize(len(el))
+        mat.C = Cfull.clone()
+    m = Solid(nodes.cuda(), el.cuda(), mat)
+    o = O.OracleModel(nodes, el, O.ELEMENTS[name], "solid", Cfull)
+    return m, o
+
+
+CASES = [("Quad1", False), ("Tetra1", False), ("Tetra1-heat", False), ("Hexa1", False),
+         ("Hexa1", True), ("Hexa2", False), ("Hexa2", True)]
+
+
+@pytest.mark.parametrize("name,general", CASES)
+def test_device_pattern_is_bit_exact(name, general):
+    m, o = _case(name, general)
+    assert torch.equal(m.glob_idx.cpu(), o.glob_idx)
+    assert torch.equal(m.k_map.cpu(), o.k_map)
+    assert torch.equal(m.diag_map.cpu(), o.diag_map)
+    assert torch.equal(m.idx.cpu().long(), o.idx)
+
+
+@pytest.mark.parametrize("name,general", CASES)
+def test_element_matrices(name, general):
+    m, o = _case(name, general)
+    k = m.k0().cpu()
+    ko = o.k0()
+    assert torch.allclose(k, ko, rtol=1e-12, atol=1e-12 * ko.abs().max())
+
+
+def _constraints(m, o):
+    con = torch.zeros(o.n_nod, o.ndof, dtype=torch.bool)
+    con[o.nodes[:, 0] == o.nodes[:, 0].min()] = True
+    con[3, 0] = True
+    m.constraints = con.cuda()
+    return con
+
+
+@pytest.mark.parametrize("name,general", CASES)
+def test_fused_assembly_matches_oracle(name, general):
+    m, o = _case(name, general)
+    con = _constraints(m, o)
+    kind, p0, p1, sc = m._material_params()
+    s64 = _System(m, kind, p0, p1, sc, "fp64")
+    s32 = _System(m, kind, p0, p1, sc, "mixed")
+    cidx = torch.nonzero(con.ravel()).ravel()
+    ref = o.assemble_matrix(o.k0(), cidx)
+    v = s64.val64.cpu()
+    assert torch.allclose(v, ref, rtol=1e-12, atol=1e-12 * ref.abs().max())
+    # the FP32 copy is the rounding of the FP64 sums
+    assert torch.equal(s32.val32.cpu(), v.float())
+    # bitwise reproducible
+    s64b = _System(m, kind, p0, p1, sc, "fp64")
+    assert torch.equal(s64b.val64, s64.val64)
+    # nodal diagonal blocks are the matrix' own diagonal blocks
+    nd = o.ndof
+    Kd = o.csr(v).toarray()
+    blocks = np.stack([Kd[n * nd:(n + 1) * nd, n * nd:(n + 1) * nd] for n in range(o.n_nod)])
+    assert np.array_equal(s64.diag.cpu().numpy().reshape(-1, nd, nd), blocks)
+    # assemble_matrix(k) from element matrices gives the same values
+    K = m.assemble_matrix(m.k0(), cidx.cuda())
+    assert torch.allclose(K._values().cpu(), v, rtol=1e-13, atol=1e-13 * ref.abs().max())
+
+
+@pytest.mark.parametrize("name,general", CASES)
+def test_spmv(name, general):
+    m, o = _case(name, general)
+    _constraints(m, o)
+    kind, p0, p1, sc = m._material_params()
+    s64 = _System(m, kind, p0, p1, sc, "fp64")
+    s32 = _System(m, kind, p0, p1, sc, "mixed")
+    A = o.csr(s64.val64.cpu())
+    x = torch.randn(o.n_dofs, generator=torch.Generator().manual_seed(2), dtype=DT)
+    ref = A @ x.numpy()
+    pat = m._pattern
+    for prec, vals, xx, tol in [(0, s64.val64, x.cuda(), 1e-13), (1, s32.val32, x.float().cuda(), 3e-6)]:
+        y = torch.empty_like(xx)
+        L.check(L.lib().tfm_spmv(pat.stream(), C.byref(pat.mesh), prec, L.ptr(vals), L.ptr(xx), L.ptr(y)))
+        err = np.abs(y.double().cpu().numpy() - ref).max() / np.abs(ref).max()
+        assert err < tol, (prec, err)
+
+
+@pytest.mark.parametrize("name,general", CASES)
+def test_internal_force_and_fluxes(name, general):
+    m, o = _case(name, general)
+    u = 1e-3 * torch.randn(o.n_dofs, generator=torch.Generator().manual_seed(4), dtype=DT)
+    kind, p0, p1, sc = m._material_params()
+    h = _MatHolder(kind, p0, p1, sc)
+    F, fe, gm, qm, gg, qg = m._forces(h.mat, u.cuda(), means=True, per_gp=True)
+    _, fo, grado, fluxo = o.integrate(u, need_k=False)
+    Fo = o.assemble_rhs(fo)
+    assert torch.allclose(F.cpu(), Fo, rtol=1e-12, atol=1e-12 * Fo.abs().max())
+    assert torch.allclose(fe.cpu().reshape(fo.shape), fo, rtol=1e-12, atol=1e-12 * fo.abs().max())
+    qo = fluxo.reshape(qg.shape)
+    assert torch.allclose(qg.cpu(), qo, rtol=1e-12, atol=1e-12 * qo.abs().max())
+    assert torch.allclose(qm.cpu(), qo.mean(0), rtol=1e-12, atol=1e-12 * qo.abs().max())
+    Ho = grado.reshape(gg.shape)
+    if kind in (L.MAT_ISO, L.MAT_GENERAL):  # mechanics grad = I + H
+        Ho = Ho - torch.eye(o.n_dim, dtype=DT).reshape(-1)
+    assert torch.allclose(gg.cpu(), Ho, rtol=1e-12, atol=1e-14)
+    # K u from the assembled (unconstrained) matrix equals the matrix-free force
+    K = o.csr(o.assemble_matrix(o.k0(), torch.zeros(0, dtype=torch.long)))
+    Ku = K @ u.numpy()
+    assert np.allclose(F.cpu().numpy(), Ku, rtol=1e-11, atol=1e-12 * np.abs(Ku).max())
+    # assemble_rhs API
+    F2 = m.assemble_rhs(fe.reshape(m.n_elem, -1))
+    assert torch.equal(F2, F)
+
+
+@pytest.mark.parametrize("name,general", CASES)
+def test_sensitivity_matches_autograd(name, general):
+    """lam^T dK/dp u per element against torch autograd of the oracle's
+    element matrices with respect to the material tangent and thickness."""
+    m, o = _case(name, general)
+    g = torch.Generator().manual_seed(5)
+    u = torch.randn(o.n_dofs, generator=g, dtype=DT)
+    lam = torch.randn(o.n_dofs, generator=g, dtype=DT)
+    kind, p0, p1, sc = m._material_params()
+    h = _MatHolder(kind, p0, p1, sc)
+    from torchfem.base import _SensView
+    out = _SensView(m, h).sensitivity(u.cuda(), lam.cuda(), -1.0)[0].cpu()
+    # autograd reference through the oracle's residual lam^T f(u; p)
+    mat = o.mat.clone().requires_grad_(True)
+    tk = o.thickness.clone().requires_grad_(True) if o.thickness is not None else None
+    oo = O.OracleModel(o.nodes, o.elements, o.etype, o.kind, mat, thickness=tk, pattern=o.pattern)
+    _, f, _, _ = oo.integrate(u, need_k=False)
+    val = -torch.inner(oo.assemble_rhs(f), lam)
+    grads = torch.autograd.grad(val, [mat] + ([tk] if tk is not None else []))
+    gC = grads[0]
+    E = o.n_elem
+    d = o.n_dim
+    if kind == L.MAT_ISO:
+        I = torch.eye(d, dtype=DT)
+        I4 = torch.einsum("ij,kl->ijkl", I, I)
+        I4S = torch.einsum("ik,jl->ijkl", I, I) + torch.einsum("il,jk->ijkl", I, I)
+        ref0 = torch.einsum("eijkl,ijkl->e", gC, I4)
+        ref1 = torch.einsum("eijkl,ijkl->e", gC, I4S)
+        assert torch.allclose(out[:, 0], ref0, rtol=1e-10, atol=1e-12 * ref0.abs().max())
+        assert torch.allclose(out[:, 1], ref1, rtol=1e-10, atol=1e-12 * ref1.abs().max())
+    elif kind == L.MAT_GENERAL:
+        ref = gC.reshape(E, -1)
+        assert torch.allclose(out[:, : d**4], ref, rtol=1e-10, atol=1e-12 * ref.abs().max())
+    elif kind == L.MAT_HEAT_ISO:
+        ref0 = torch.einsum("eij,ij->e", gC, torch.eye(d, dtype=DT))
+        assert torch.allclose(out[:, 0], ref0, rtol=1e-10, atol=1e-12 * ref0.abs().max())
+    if tk is not None:
+        assert torch.allclose(out[:, -1], grads[1], rtol=1e-10, atol=1e-12 * grads[1].abs().max())

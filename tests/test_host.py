@@ -1,0 +1,112 @@
+"""CPU tests of the host side: the C-ABI library loads and exports every
+entry point declared in include/tfm.h, and the host pattern builder
+reproduces the reference's index arrays bit for bit (no GPU needed)."""
+
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from torchfem import _lib as L
+from torchfem._pattern import build_host_pattern
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "tfm.h").read_text()
+    declared = set(re.findall(r"\b(tfm_[a-z0-9_]+)\s*\(", header))
+    assert len(declared) >= 19
+    lib = L.lib()
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    assert lib.tfm_version() == 1
+
+
+def test_signature_table_covers_the_header():
+    header = (ROOT / "include" / "tfm.h").read_text()
+    declared = set(re.findall(r"\b(tfm_[a-z0-9_]+)\s*\(", header))
+    assert declared == set(L._SIGNATURES)
+
+
+def test_pattern_builder_rejects_bad_connectivity():
+    el = torch.tensor([[0, 1, 2, 7]])
+    with pytest.raises(L.HIPError, match="out of range"):
+        build_host_pattern(el, 4, 2)
+
+
+def _expand(hp, elements, ndof):
+    """The documented node-block layout expanded on the host: entry (a, p, b)
+    of node n is row n*ndof+a, column adj[p]*ndof+b at val_ptr[n]+a*ndof*deg+p*ndof+b."""
+    n_nod = len(hp["adj_ptr"]) - 1
+    rows = np.empty(hp["nnz"], np.int64)
+    cols = np.empty(hp["nnz"], np.int64)
+    rowstart = np.empty(n_nod * ndof, np.int64)
+    for n in range(n_nod):
+        a0, a1 = hp["adj_ptr"][n], hp["adj_ptr"][n + 1]
+        deg = a1 - a0
+        vb = hp["val_ptr"][n]
+        if deg == 0:
+            for a in range(ndof):
+                rows[vb + a] = cols[vb + a] = n * ndof + a
+                rowstart[n * ndof + a] = vb + a
+            continue
+        adj = hp["adj"][a0:a1]
+        for a in range(ndof):
+            rowstart[n * ndof + a] = vb + a * ndof * deg
+            q = vb + a * ndof * deg + np.arange(ndof * deg)
+            rows[q] = n * ndof + a
+            cols[q] = (adj[:, None] * ndof + np.arange(ndof)[None, :]).ravel()
+    el = elements.numpy()
+    E, npe = el.shape
+    epos = hp["epos"].reshape(E, npe, npe).astype(np.int64)
+    kmap = (rowstart[(el[:, :, None] * ndof + np.arange(ndof)).reshape(E, npe * ndof)][:, :, None]
+            + (np.repeat(epos, ndof, axis=1).repeat(ndof, axis=2) * ndof)
+            + np.tile(np.arange(ndof), npe)[None, None, :])
+    diag = np.empty(n_nod * ndof, np.int64)
+    for n in range(n_nod):
+        a0, a1 = hp["adj_ptr"][n], hp["adj_ptr"][n + 1]
+        deg = a1 - a0
+        for a in range(ndof):
+            if deg == 0:
+                diag[n * ndof + a] = rowstart[n * ndof + a]
+            else:
+                p = int(np.searchsorted(hp["adj"][a0:a1], n))
+                diag[n * ndof + a] = rowstart[n * ndof + a] + p * ndof + a
+    return rows, cols, kmap.reshape(-1), diag
+
+
+@pytest.mark.parametrize(
+    "mesh,ndof",
+    [
+        (lambda: O.cube_hexa(4, 3, 5), 3),
+        (lambda: O.rect_quad(6, 4), 2),
+        (lambda: O.cube_tetra6(3), 1),
+        (lambda: O.linear_to_quadratic(*O.cube_hexa(3, 3, 3)), 3),
+        (lambda: (torch.cat([O.rect_quad(3, 3)[0], torch.tensor([[5.0, 5.0]])]), O.rect_quad(3, 3)[1]), 2),
+    ],
+)
+def test_host_pattern_reproduces_reference_indices(mesh, ndof):
+    nodes, el = mesh()
+    hp = build_host_pattern(el, len(nodes), ndof)
+    _, glob_idx, k_map, diag_map = O.build_pattern(el, len(nodes), ndof)
+    rows, cols, kmap, diag = _expand(hp, el, ndof)
+    assert np.array_equal(rows, glob_idx[0].numpy())
+    assert np.array_equal(cols, glob_idx[1].numpy())
+    assert np.array_equal(kmap, k_map.numpy().astype(np.int64))
+    assert np.array_equal(diag, diag_map.numpy().astype(np.int64))
+    # incidence lists are ascending by element id (the gather order)
+    for n in range(len(nodes)):
+        s = hp["ne_elem"][hp["ne_ptr"][n]:hp["ne_ptr"][n + 1]]
+        assert np.all(np.diff(s) >= 0)
+
+
+def test_hexa_cube_64_pattern_size_closed_form():
+    """At the benchmark size the host pattern has 9 (3n-2)^3 entries (n = 65)."""
+    nodes, el = O.cube_hexa(65, 65, 65)
+    hp = build_host_pattern(el, len(nodes), 3)
+    assert hp["nnz"] == O.hexa_nnz_closed_form(65) == 64_701_513
+    assert hp["max_deg"] == 27 and hp["max_inc"] == 8

@@ -1,0 +1,368 @@
+"""Pins the CPU oracle (oracle/fem_oracle.py) before it is trusted as the
+checker of the HIP path. Each test fixes the oracle by something other than
+its own formulas: the reference's own known-answer tests and golden value,
+closed forms, exact symbolic integration, invariants and finite differences.
+"""
+
+import itertools
+import math
+
+import pytest
+import sympy
+import torch
+
+import oracle as O
+
+DT = torch.float64
+
+
+# ---------------------------------------------------------------- elements
+# tests/test_elements.py:25-66 of the reference, applied to the restated tables
+@pytest.mark.parametrize("name", ["Quad1", "Tetra1", "Hexa1", "Hexa2"])
+def test_quadrature_volume(name):
+    el = O.ELEMENTS[name]
+    vol = 0.0
+    for w, q in zip(el.iweights, el.ipoints):
+        J = el.B(q) @ el.iso_coords
+        vol += float(w) * float(torch.linalg.det(J))
+    assert math.isclose(vol, el.iso_volume, rel_tol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["Quad1", "Tetra1", "Hexa1", "Hexa2"])
+def test_shape_gradient_is_derivative_of_N(name):
+    el = O.ELEMENTS[name]
+    for q in el.ipoints:
+        q = q.clone().requires_grad_(True)
+        for i in range(el.nodes):
+            (g,) = torch.autograd.grad(el.N(q)[i], q)
+            assert torch.allclose(g, el.B(q.detach())[:, i], atol=1e-13)
+
+
+@pytest.mark.parametrize("name", ["Quad1", "Tetra1", "Hexa1", "Hexa2"])
+def test_kronecker_property(name):
+    el = O.ELEMENTS[name]
+    assert torch.allclose(el.N(el.iso_coords), torch.eye(el.nodes, dtype=DT), atol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["Quad1", "Tetra1", "Hexa1", "Hexa2"])
+def test_weights_sum_to_iso_volume(name):
+    el = O.ELEMENTS[name]
+    assert math.isclose(float(el.iweights.sum()), el.iso_volume, rel_tol=1e-12)
+
+
+# ---------------------------------------------------------------- meshes
+def test_cube_hexa_numbering():
+    """mesh.py:8-46: node i*Ny*Nz + j*Nz + k at (x_i, y_j, z_k)."""
+    nodes, el = O.cube_hexa(3, 4, 5)
+    assert nodes.shape == (60, 3) and el.shape == (2 * 3 * 4, 8)
+    assert torch.allclose(nodes[1 * 20 + 2 * 5 + 3], torch.tensor([0.5, 2 / 3, 0.75], dtype=DT), rtol=0, atol=1e-15)
+    # first element: [n0, n1, n3, n2, n4, n5, n7, n6] with n1 = +x, n2 = +y, n4 = +z
+    assert el[0].tolist() == [0, 20, 25, 5, 1, 21, 26, 6]
+
+
+def test_cube_tetra6_is_conforming_and_positive():
+    nodes, el = O.cube_tetra6(3, jitter=0.1, seed=0)
+    m = O.OracleModel(nodes, el, O.Tetra1, "heat", O.iso_KAPPA(torch.ones(len(el))))
+    _, _, detJ = m.eval_shape_functions(O.Tetra1.ipoints)
+    assert (detJ > 0).all()
+    assert math.isclose(float((detJ / 6.0).sum()), 1.0, rel_tol=1e-12)  # total volume
+    faces = el[:, O.Tetra1.facets].reshape(-1, 3).sort(dim=1).values
+    _, counts = torch.unique(faces, dim=0, return_counts=True)
+    assert counts.max() == 2  # every face is shared by at most two tets
+    n_boundary = int((counts == 1).sum())
+    assert n_boundary == 6 * 2 * 3 * 3  # 2 triangles per boundary quad
+
+
+def test_linear_to_quadratic_counts():
+    nodes, el = O.linear_to_quadratic(*O.cube_hexa(3, 3, 3))
+    # 27 vertices + 3*3*3*... edges: 3 * (2 * 3 * 3) = 54 edges
+    assert nodes.shape[0] == 27 + 54 and el.shape == (8, 20)
+    mid = nodes[el[:, 8:]]
+    iso = O.Hexa2.iso_coords[8:]
+    corners = nodes[el[:, :8]]
+    # every mid-side node is the midpoint of its two corners
+    pairs = torch.tensor([[0, 1], [1, 2], [2, 3], [3, 0], [4, 5], [5, 6], [6, 7], [7, 4],
+                          [0, 4], [1, 5], [2, 6], [3, 7]])
+    assert torch.allclose(mid, 0.5 * (corners[:, pairs[:, 0]] + corners[:, pairs[:, 1]]))
+    assert iso.shape == (12, 3)
+
+
+# ---------------------------------------------------------------- pattern
+@pytest.mark.parametrize("n", [2, 3, 4, 5])
+def test_hexa_pattern_nnz_closed_form(n):
+    nodes, el = O.cube_hexa(n, n, n)
+    _, glob_idx, _, _ = O.build_pattern(el, len(nodes), 3)
+    assert glob_idx.shape[1] == O.hexa_nnz_closed_form(n)
+
+
+@pytest.mark.parametrize(
+    "mesh,ndof",
+    [
+        (lambda: O.cube_hexa(3, 4, 3), 3),
+        (lambda: O.rect_quad(4, 3), 2),
+        (lambda: O.cube_tetra6(2), 1),
+        (lambda: O.linear_to_quadratic(*O.cube_hexa(3, 3, 2)), 3),
+    ],
+)
+def test_pattern_matches_brute_force(mesh, ndof):
+    """glob_idx is the sorted set of element DOF couplings plus the diagonal,
+    recomputed here from a dense incidence product."""
+    nodes, el = mesh()
+    n = len(nodes) * ndof
+    idx, glob_idx, k_map, diag_map = O.build_pattern(el, len(nodes), ndof)
+    A = torch.zeros(n, n, dtype=torch.bool)
+    for row in idx:
+        A[row[:, None], row[None, :]] = True
+    A |= torch.eye(n, dtype=torch.bool)
+    expect = A.nonzero().T  # row-major order
+    assert torch.equal(glob_idx, expect)
+    # k_map sends each element entry to the slot of its (row, col)
+    rows = idx[:, :, None].expand(-1, -1, idx.shape[1]).reshape(-1)
+    cols = idx[:, None, :].expand(-1, idx.shape[1], -1).reshape(-1)
+    assert torch.equal(glob_idx[0, k_map.long()], rows)
+    assert torch.equal(glob_idx[1, k_map.long()], cols)
+    assert torch.equal(glob_idx[0, diag_map.long()], torch.arange(n))
+    assert torch.equal(glob_idx[1, diag_map.long()], torch.arange(n))
+
+
+def test_pattern_orphan_node_keeps_only_its_diagonal():
+    nodes, el = O.rect_quad(3, 3)
+    nodes = torch.cat([nodes, torch.tensor([[5.0, 5.0]], dtype=DT)])
+    _, glob_idx, _, _ = O.build_pattern(el, len(nodes), 2)
+    last = glob_idx[:, glob_idx[0] >= 18]
+    assert last.tolist() == [[18, 19], [18, 19]]
+
+
+# ---------------------------------------------------------------- materials
+def test_iso_C_lame_and_symmetries():
+    """tests/test_materials.py:89-131: C symmetric, Lame constants."""
+    E, nu = 1000.0, 0.3
+    C = O.iso3d_C(torch.tensor(E), torch.tensor(nu))
+    lbd = E * nu / ((1 + nu) * (1 - 2 * nu))
+    G = E / (2 * (1 + nu))
+    assert math.isclose(float(C[0, 0, 1, 1]), lbd)
+    assert math.isclose(float(C[0, 1, 0, 1]), G)
+    assert math.isclose(float(C[0, 0, 0, 0]), lbd + 2 * G)
+    assert torch.allclose(C, C.permute(2, 3, 0, 1))
+    assert torch.allclose(C, C.permute(1, 0, 2, 3))
+
+
+def test_plane_stress_C_closed_form():
+    E, nu = 210.0, 0.25
+    C = O.plane_stress_C(torch.tensor(E), torch.tensor(nu))
+    sig_xx = C[0, 0, 0, 0] * 1.0  # uniaxial strain eps_xx = 1 with sigma_zz = 0
+    assert math.isclose(float(sig_xx), E / (1 - nu**2))
+    assert math.isclose(float(C[0, 1, 0, 1]), E / (2 * (1 + nu)))
+
+
+# ---------------------------------------------------------------- element matrices
+def _unit_cube_hexa_ke_symbolic(E, nu):
+    """Exact Ke of a unit-cube Hexa1 by symbolic integration of B^T C B,
+    with the trilinear functions built from the corner coordinates only."""
+    x, y, z = sympy.symbols("x y z")
+    corners = O.Hexa1.iso_coords.tolist()
+    N = []
+    for c in corners:
+        f = 1
+        for v, s in zip((x, y, z), c):
+            f *= v if s > 0 else (1 - v)
+        N.append(f)
+    lam = sympy.Rational(E) * sympy.Rational(nu) / ((1 + sympy.Rational(nu)) * (1 - 2 * sympy.Rational(nu)))
+    G = sympy.Rational(E) / (2 * (1 + sympy.Rational(nu)))
+    grads = [[sympy.diff(Ni, v) for v in (x, y, z)] for Ni in N]
+    K = sympy.zeros(24, 24)
+    for p, q in itertools.product(range(8), range(8)):
+        bp, bq = grads[p], grads[q]
+        for i, k in itertools.product(range(3), range(3)):
+            expr = lam * bp[i] * bq[k] + G * bp[k] * bq[i] + (G * sum(bp[j] * bq[j] for j in range(3)) if i == k else 0)
+            K[3 * p + i, 3 * q + k] = sympy.integrate(sympy.expand(expr), (x, 0, 1), (y, 0, 1), (z, 0, 1))
+    return torch.tensor([[float(K[i, j]) for j in range(24)] for i in range(24)], dtype=DT)
+
+
+def test_unit_cube_hexa1_ke_matches_exact_integration():
+    nodes, el = O.cube_hexa(2, 2, 2)
+    m = O.OracleModel(nodes, el, O.Hexa1, "solid", O.iso3d_C(torch.tensor([1000.0]), 0.3))
+    k = m.k0()[0]
+    ref = _unit_cube_hexa_ke_symbolic(1000, sympy.Rational(3, 10))
+    assert torch.allclose(k, ref, rtol=1e-12, atol=1e-10)
+
+
+def _affine_mesh(name):
+    g = torch.Generator().manual_seed(3)
+    A = torch.eye(3, dtype=DT) + 0.2 * torch.rand(3, 3, generator=g, dtype=DT)
+    if name == "Hexa1":
+        nodes, el = O.cube_hexa(3, 2, 2)
+    elif name == "Hexa2":
+        nodes, el = O.linear_to_quadratic(*O.cube_hexa(2, 3, 2))
+    else:
+        nodes, el = O.cube_tetra6(2)
+    return nodes @ A.T, el
+
+
+@pytest.mark.parametrize("name", ["Hexa1", "Hexa2", "Tetra1"])
+def test_ke_energy_of_linear_field_and_rigid_modes(name):
+    """u = G x gives u^T Ke u = V eps:C:eps exactly (constant strain on an
+    affine element); rigid motions give Ke u = 0; Ke is symmetric."""
+    nodes, el = _affine_mesh(name)
+    E = torch.linspace(500.0, 1500.0, len(el), dtype=DT)
+    m = O.OracleModel(nodes, el, O.ELEMENTS[name], "solid", O.iso3d_C(E, 0.3))
+    k = m.k0()
+    _, _, detJ = m.eval_shape_functions(m.etype.ipoints)
+    vol = torch.einsum("i,ie->e", m.etype.iweights.to(DT), detJ)
+    Gm = torch.tensor([[1e-3, 2e-4, -1e-4], [3e-4, -1e-3, 5e-5], [1e-4, 2e-4, 7e-4]], dtype=DT)
+    ue = (nodes[el] @ Gm.T).reshape(len(el), -1)
+    eps = 0.5 * (Gm + Gm.T)
+    lbd = E * 0.3 / (1.3 * 0.4)
+    G = E / 2.6
+    dens = lbd * torch.trace(eps) ** 2 + 2 * G * (eps * eps).sum()
+    assert torch.allclose(torch.einsum("ei,eij,ej->e", ue, k, ue), vol * dens, rtol=1e-11)
+    assert torch.allclose(k, k.transpose(1, 2), atol=1e-10)
+    W = torch.tensor([[0.0, -0.3, 0.2], [0.3, 0.0, -0.1], [-0.2, 0.1, 0.0]], dtype=DT)
+    urig = (nodes[el] @ W.T + torch.tensor([0.1, -0.2, 0.3], dtype=DT)).reshape(len(el), -1)
+    assert torch.einsum("eij,ej->ei", k, urig).abs().max() < 1e-9
+
+
+def test_heat_k0_symmetric_zero_row_sums_scales():
+    """tests/test_base.py:108-131."""
+    nodes, el = O.cube_tetra6(2, jitter=0.1)
+    kap = torch.full((len(el),), 400.0, dtype=DT)
+    m = O.OracleModel(nodes, el, O.Tetra1, "heat", O.iso_KAPPA(kap))
+    k = m.k0()
+    assert torch.allclose(k, k.transpose(1, 2))
+    assert torch.allclose(k.sum(-1), torch.zeros(len(el), 4, dtype=DT), atol=1e-9)
+    m2 = O.OracleModel(nodes, el, O.Tetra1, "heat", O.iso_KAPPA(2 * kap), pattern=m.pattern)
+    assert torch.allclose(m2.k0(), 2 * k)
+
+
+# ---------------------------------------------------------------- solves
+def _boundary(nodes):
+    mask = torch.zeros(len(nodes), dtype=torch.bool)
+    for d in range(nodes.shape[1]):
+        c = nodes[:, d]
+        mask |= torch.isclose(c, c.min()) | torch.isclose(c, c.max())
+    return mask
+
+
+@pytest.mark.parametrize("name", ["Quad1", "Tetra1", "Hexa1", "Hexa2"])
+def test_patch_test(name):
+    """tests/test_models.py:54-71: a linear field is reproduced exactly."""
+    if name == "Quad1":
+        nodes, el = O.rect_quad(4, 4)
+        m = O.OracleModel(nodes, el, O.Quad1, "planar", O.plane_stress_C(torch.full((len(el),), 1000.0), 0.3),
+                          thickness=torch.ones(len(el), dtype=DT))
+        Gm = torch.tensor([[1.0e-3, 2.0e-4], [3.0e-4, -1.0e-3]], dtype=DT)
+    else:
+        if name == "Tetra1":
+            nodes, el = O.cube_tetra6(2)
+        elif name == "Hexa1":
+            nodes, el = O.cube_hexa(3, 3, 3)
+        else:
+            nodes, el = O.linear_to_quadratic(*O.cube_hexa(3, 3, 3))
+        m = O.OracleModel(nodes, el, O.ELEMENTS[name], "solid", O.iso3d_C(torch.full((len(el),), 1000.0), 0.3))
+        Gm = torch.tensor([[1.0e-3, 2.0e-4, -1.0e-4], [3.0e-4, -1.0e-3, 5.0e-5], [1.0e-4, 2.0e-4, 7.0e-4]], dtype=DT)
+    u_exact = nodes @ Gm.T
+    con = _boundary(nodes)[:, None].repeat(1, m.ndof)
+    u, f, sigma, _ = m.solve(con, torch.zeros_like(nodes), u_exact)
+    assert torch.allclose(u, u_exact, atol=1e-12)
+    assert torch.allclose(sigma, sigma[0].expand_as(sigma), atol=1e-12)
+
+
+def test_solve_matches_dense_solution():
+    """tests/test_sparse.py:49-56: the sparse solve agrees with a dense one."""
+    nodes, el = O.cube_hexa(3, 3, 3)
+    m = O.OracleModel(nodes, el, O.Hexa1, "solid", O.iso3d_C(torch.full((len(el),), 1000.0), 0.3))
+    con = torch.zeros(len(nodes), 3, dtype=torch.bool)
+    con[nodes[:, 0] == 0.0] = True
+    k = m.k0()
+    val = m.assemble_matrix(k, torch.nonzero(con.ravel()).ravel())
+    K = torch.tensor(m.csr(val).toarray())
+    b = torch.randn(m.n_dofs, generator=torch.Generator().manual_seed(0), dtype=DT)
+    b[con.ravel()] = 0.0
+    x_ref = torch.linalg.solve(K, b)
+    x_cg = torch.as_tensor(m.solve_linear(m.csr(val), b.numpy(), method="cg", rtol=1e-14))
+    x_lu = torch.as_tensor(m.solve_linear(m.csr(val), b.numpy(), method="spsolve"))
+    assert torch.allclose(x_cg, x_ref, atol=1e-10)
+    assert torch.allclose(x_lu, x_ref, atol=1e-12)
+
+
+def _readme_cantilever(thickness):
+    nodes = torch.tensor([[0.0, 0.0], [1.0, 0.0], [2.0, 0.0], [0.0, 1.0], [1.0, 1.0], [2.0, 1.0]], dtype=DT)
+    el = torch.tensor([[0, 1, 4, 3], [1, 2, 5, 4]])
+    C = O.plane_stress_C(torch.full((2,), 1000.0, dtype=DT), torch.tensor(0.3, dtype=DT))
+    m = O.OracleModel(nodes, el, O.Quad1, "planar", C, thickness=thickness)
+    con = torch.zeros(6, 2, dtype=torch.bool)
+    con[[0, 3], :] = True
+    F = torch.zeros(6, 2, dtype=DT)
+    F[5, 1] = -1.0
+    return m, con, F
+
+
+def test_readme_golden_thickness_gradient():
+    """README.md:101-113: dC/dthickness = [-0.0208, -0.0053] (4 decimals)."""
+    t = torch.ones(2, dtype=DT, requires_grad=True)
+    m, con, F = _readme_cantilever(t)
+    u, f, _, _ = m.solve(con, F, torch.zeros(6, 2, dtype=DT), params=(t,))
+    compliance = torch.inner(f.ravel(), u.ravel())
+    (g,) = torch.autograd.grad(compliance, t)
+    assert torch.allclose(g, torch.tensor([-0.0208, -0.0053], dtype=DT), atol=5e-5)
+    # and central finite differences of the same objective
+    h = 1e-6
+    for e in range(2):
+        vals = []
+        for s in (1, -1):
+            tt = torch.ones(2, dtype=DT)
+            tt[e] += s * h
+            m2, _, _ = _readme_cantilever(tt)
+            u2, f2, _, _ = m2.solve(con, F, torch.zeros(6, 2, dtype=DT))
+            vals.append(float(torch.inner(f2.ravel(), u2.ravel())))
+        assert math.isclose(float(g[e]), (vals[0] - vals[1]) / (2 * h), rel_tol=1e-6)
+
+
+def _simp_cube(n, rho):
+    nodes, el = O.cube_hexa(n + 1, n + 1, n + 1)
+    C = O.iso3d_C(1000.0 * rho**3, torch.tensor(0.3, dtype=DT))
+    m = O.OracleModel(nodes, el, O.Hexa1, "solid", C)
+    con = torch.zeros(len(nodes), 3, dtype=torch.bool)
+    con[nodes[:, 0] == 0.0] = True
+    F = torch.zeros(len(nodes), 3, dtype=DT)
+    right = nodes[:, 0] == 1.0
+    F[right, 2] = -1.0 / int(right.sum())
+    return m, con, F
+
+
+def test_simp_compliance_gradient_closed_form_and_fd():
+    """Implicit-function gradient vs the self-adjoint closed form
+    -3 rho^2 E0 u_e^T k0_e u_e (topology.ipynb cell 8) and central FD."""
+    n = 3
+    g = torch.Generator().manual_seed(0)
+    rho = (0.3 + 0.7 * torch.rand(n**3, generator=g, dtype=DT)).requires_grad_(True)
+    m, con, F = _simp_cube(n, rho)
+    u, f, _, _ = m.solve(con, F, torch.zeros_like(F), params=(rho,))
+    comp = torch.inner(F.ravel(), u.ravel())
+    (grad,) = torch.autograd.grad(comp, rho)
+    unit = O.OracleModel(m.nodes, m.elements, O.Hexa1, "solid",
+                         O.iso3d_C(torch.ones(n**3, dtype=DT), torch.tensor(0.3, dtype=DT)),
+                         pattern=m.pattern)
+    closed = O.simp_compliance_sensitivity(unit, u.detach(), rho.detach(), 1000.0)
+    assert torch.allclose(grad, closed, rtol=1e-8)
+    h = 1e-6
+    for e in [0, 7, 13, 26]:
+        vals = []
+        for s in (1, -1):
+            r2 = rho.detach().clone()
+            r2[e] += s * h
+            m2, _, _ = _simp_cube(n, r2)
+            u2, _, _, _ = m2.solve(con, F, torch.zeros_like(F))
+            vals.append(float(torch.inner(F.ravel(), u2.ravel())))
+        assert math.isclose(float(grad[e]), (vals[0] - vals[1]) / (2 * h), rel_tol=1e-5)
+
+
+def test_surface_load_totals():
+    """tests/test_loads.py:118-160: a uniform traction integrates to traction x area."""
+    nodes, el = O.linear_to_quadratic(*O.cube_hexa(3, 3, 3))
+    m = O.OracleModel(nodes, el, O.Hexa2, "solid", O.iso3d_C(torch.full((len(el),), 1.0), 0.3))
+    top = nodes[:, 2] == 1.0
+    load, conn = m.integrate_surface_load(top, torch.tensor([0.3, -0.2, 1.0], dtype=DT))
+    assert conn.shape == (4, 8)
+    assert torch.allclose(load.sum(0), torch.tensor([0.3, -0.2, 1.0], dtype=DT), atol=1e-13)
+    assert (load[~top] == 0).all()

@@ -1,0 +1,852 @@
+"""FEM core: the assemble-and-solve path on the GPU (src/torchfem/base.py).
+
+`FEM.solve()` keeps the reference's signature and outputs (base.py:597-923)
+for linear, stateless materials. The work runs in libtfm_hip.so:
+
+* pattern: host-built node adjacency (FEM.__init__, base.py:72-122);
+* integration + assembly: one fused kernel writes the CSR values with the
+  Dirichlet epilogue (integrate_material + assemble_matrix, base.py:398-426,
+  982-1092, 1177-1286), deterministic, no float atomics;
+* solve: Jacobi / block-Jacobi PCG, all-FP64 or FP32-inner with FP64
+  iterative refinement on the matrix-free FP64 residual (sparse_solve,
+  sparse.py:271-444);
+* gradients: a hand-written backward (NewtonRaphsonAdjoint.backward,
+  sparse.py:641-724): the adjoint reuses the PCG and a fused kernel forms
+  -lam_e^T (dK_e/dp_e) u_e for every element with K_e recomputed.
+
+For a linear material the Newton loop of the reference converges after one
+linear solve per load state; the implementation keeps that loop (residual
+check with rtol/atol, further corrections if needed) on the total
+displacement, and scales the unit-load solution to the requested increments,
+which is exact for a linear model.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from abc import ABC, abstractmethod
+from collections.abc import Iterable
+from functools import cached_property
+
+import torch
+from torch import Tensor
+
+from . import _lib as L
+from ._pattern import DevicePattern
+from .elements import Element
+from .materials import Material
+from .sparse import CachedSolve, PCGSolver, default_precision
+
+_PREC = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED}
+_PC = {"jacobi": L.PC_JACOBI, "block_jacobi": L.PC_BLOCK_JACOBI}
+
+
+def _n_out(kind: int, dim: int) -> int:
+    return {L.MAT_ISO: 3, L.MAT_GENERAL: dim**4 + 1, L.MAT_HEAT_ISO: 2,
+            L.MAT_HEAT_GENERAL: dim**2 + 1}[kind]
+
+
+class _System:
+    """Assembled operator of one solve: CSR values (FP64 and/or FP32), nodal
+    diagonal blocks, the Dirichlet mask and the material it was built from."""
+
+    def __init__(self, fem: "FEM", kind: int, p0: Tensor, p1: Tensor | None,
+                 scale: Tensor | None, precision: str):
+        pat = fem._pattern
+        dev = pat.device
+        self.fem = fem
+        self.kind = kind
+        self.p0 = p0.detach().contiguous()
+        self.p1 = p1.detach().contiguous() if p1 is not None else None
+        self.scale = scale.detach().contiguous() if scale is not None else None
+        self.mat = L.Material(kind=kind, p0=self.p0.data_ptr(),
+                              p1=self.p1.data_ptr() if self.p1 is not None else None,
+                              scale=self.scale.data_ptr() if self.scale is not None else None)
+        self.con = fem._con_mask()
+        self.precision = precision
+        self.val64 = torch.empty(pat.nnz, dtype=torch.float64, device=dev) if precision == "fp64" else None
+        self.val32 = torch.empty(pat.nnz, dtype=torch.float32, device=dev) if precision == "mixed" else None
+        nd = pat.ndof
+        self.diag = torch.empty(pat.n_nod * nd * nd, dtype=torch.float64, device=dev)
+        bad = torch.zeros(1, dtype=torch.int32, device=dev)
+        L.check(L.lib().tfm_assemble(pat.stream(), C.byref(pat.mesh), C.byref(self.mat),
+                                     L.ptr(self.con), L.ptr(self.val64), L.ptr(self.val32),
+                                     L.ptr(self.diag), L.ptr(bad)))
+        fem._check_jacobian(bad)
+        self.sys = L.System(mesh=C.pointer(pat.mesh), mat=C.pointer(self.mat),
+                            con=self.con.data_ptr(),
+                            val64=L.ptr(self.val64), val32=L.ptr(self.val32),
+                            diag_blocks=self.diag.data_ptr())
+
+    def force(self, u: Tensor) -> Tensor:
+        """K u with K unconstrained (matrix-free, FP64)."""
+        return self.fem._forces(self.mat, u)[0]
+
+    def sensitivity(self, u: Tensor, lam: Tensor, sign: float) -> Tensor:
+        pat = self.fem._pattern
+        u = u.reshape(-1, pat.n_dofs).contiguous()
+        lam = lam.reshape(-1, pat.n_dofs).contiguous()
+        nr = u.shape[0]
+        out = torch.empty(nr, pat.n_elem, _n_out(self.kind, pat.dim), dtype=torch.float64,
+                          device=pat.device)
+        L.check(L.lib().tfm_sensitivity(pat.stream(), C.byref(pat.mesh), C.byref(self.mat),
+                                        L.ptr(u), L.ptr(lam), nr, sign, L.ptr(out)))
+        return out
+
+    def param_grads(self, out: Tensor, needs: tuple[bool, bool, bool]):
+        """Split sensitivity outputs [E, n_out] into grads of (p0, p1, scale)."""
+        d = self.fem._pattern.dim
+        g0 = g1 = gs = None
+        if self.kind == L.MAT_ISO:
+            g0, g1, gs = out[:, 0], out[:, 1], out[:, 2]
+        elif self.kind == L.MAT_HEAT_ISO:
+            g0, gs = out[:, 0], out[:, 1]
+        elif self.kind == L.MAT_GENERAL:
+            g0, gs = out[:, : d**4], out[:, d**4]
+        else:
+            g0, gs = out[:, : d**2], out[:, d**2]
+        return (g0 if needs[0] else None, g1 if needs[1] else None, gs if needs[2] else None)
+
+
+class _StaticSolve(torch.autograd.Function):
+    """u = K(p)^{-1} [F_ext on free DOFs; U_D on constrained DOFs], with the
+    implicit-function backward of NewtonRaphsonAdjoint (sparse.py:641-724):
+    K^T lam = grad_u (free part), dL/dp = -lam^T d(K u)/dp, dL/dF = lam,
+    dL/dU_D = grad_u - (K lam) on constrained DOFs."""
+
+    @staticmethod
+    def forward(ctx, fem, cfg, F_ext, U_D, p0, p1, scale):
+        system = _System(fem, cfg["kind"], p0, p1, scale, cfg["precision"])
+        con = system.con.bool()
+        u0 = torch.where(con, U_D, torch.zeros_like(U_D))
+        d = fem._newton(system, cfg, F_ext, u0, con)
+        u = u0 + d
+        ctx.fem, ctx.cfg, ctx.system = fem, cfg, system
+        ctx.save_for_backward(u)
+        fem._last_system = system
+        return u
+
+    @staticmethod
+    def backward(ctx, g):
+        (u,) = ctx.saved_tensors
+        fem, cfg, system = ctx.fem, ctx.cfg, ctx.system
+        con = system.con.bool()
+        g = g.contiguous()
+        g_free = torch.where(con, torch.zeros_like(g), g)
+        t0 = time.perf_counter()
+        lam = fem._pcg(system, g_free.unsqueeze(0), cfg, warm=cfg.get("x0_adjoint"))[0]
+        if cfg.get("cached") is not None and cfg.get("update_cache"):
+            cfg["cached"].update_grad(lam)
+        fem.last_solve_info["adjoint"] = dict(fem._last_pcg, time=time.perf_counter() - t0)
+        needs = ctx.needs_input_grad
+        gF = lam if needs[2] else None
+        gD = None
+        if needs[3]:
+            Klam = system.force(lam)
+            gD = torch.where(con, g - Klam, torch.zeros_like(g))
+        g0 = g1 = gs = None
+        if any(needs[4:7]):
+            out = system.sensitivity(u, lam, -1.0)[0]
+            g0, g1, gs = system.param_grads(out, needs[4:7])
+        return None, None, gF, gD, g0, g1, gs
+
+
+class _Integrate(torch.autograd.Function):
+    """Converged-state evaluation (base.py:844-867): internal forces f = K u
+    (reactions included), Gauss-point mean gradient and flux."""
+
+    @staticmethod
+    def forward(ctx, fem, kind, u, p0, p1, scale):
+        mat_holder = _MatHolder(kind, p0, p1, scale)
+        F, _, gm, qm, _, _ = fem._forces(mat_holder.mat, u, means=True)
+        ctx.fem, ctx.holder = fem, mat_holder
+        ctx.save_for_backward(u)
+        return F, gm, qm
+
+    @staticmethod
+    def backward(ctx, gF, ggrad, gflux):
+        (u,) = ctx.saved_tensors
+        fem, h = ctx.fem, ctx.holder
+        needs = ctx.needs_input_grad
+        if (ggrad is not None and bool(ggrad.abs().max() > 0)) or (
+            gflux is not None and bool(gflux.abs().max() > 0)
+        ):
+            raise NotImplementedError(
+                "gradients through the stress/flux or deformation-gradient outputs are "
+                "not part of this build; differentiate u or f"
+            )
+        gu = g0 = g1 = gs = None
+        if gF is None:
+            return None, None, gu, g0, g1, gs
+        gF = gF.contiguous()
+        if needs[2]:
+            gu = fem._forces(h.mat, gF)[0]
+        if any(needs[3:6]):
+            sysv = _SensView(fem, h)
+            out = sysv.sensitivity(u, gF, 1.0)[0]
+            g0, g1, gs = sysv.param_grads(out, needs[3:6])
+        return None, None, gu, g0, g1, gs
+
+
+class _MatHolder:
+    def __init__(self, kind, p0, p1, scale):
+        self.kind = kind
+        self.p0 = p0.detach().contiguous()
+        self.p1 = p1.detach().contiguous() if p1 is not None else None
+        self.scale = scale.detach().contiguous() if scale is not None else None
+        self.mat = L.Material(kind=kind, p0=self.p0.data_ptr(),
+                              p1=self.p1.data_ptr() if self.p1 is not None else None,
+                              scale=self.scale.data_ptr() if self.scale is not None else None)
+
+
+class _SensView(_System):
+    """Sensitivity helpers for a material without an assembled matrix."""
+
+    def __init__(self, fem, holder):  # noqa: D401 - no assembly here
+        self.fem = fem
+        self.kind = holder.kind
+        self.mat = holder.mat
+
+
+class FEM(ABC):
+    """Finite-element model on one GPU (base.py:23-923)."""
+
+    def __init__(self, nodes: Tensor, elements: Tensor, material: Material | None):
+        device = L.require_gpu(nodes.device if nodes.is_cuda else None)
+        self.device = device
+        self.nodes = nodes.to(device=device, dtype=torch.float64)
+        self.elements = elements.to(device=device, dtype=torch.int64)
+        self.n_dofs = self.n_dof_per_node * nodes.shape[0]
+        self.n_nod = nodes.shape[0]
+        self.n_dim = nodes.shape[1]
+        self.n_elem = len(self.elements)
+        self.n_int = len(self.etype.iweights)
+
+        dt = torch.float64
+        self._neumann = torch.zeros(self.n_nod, self.n_dof_per_node, dtype=dt, device=device)
+        self._dirichlet = torch.zeros(self.n_nod, self.n_dof_per_node, dtype=dt, device=device)
+        self._constraints = torch.zeros(self.n_nod, self.n_dof_per_node, dtype=torch.bool,
+                                        device=device)
+        self._external_gradient = torch.zeros(self.n_elem, *self.n_flux, dtype=dt, device=device)
+
+        # local-to-global DOF map, node-major / component-minor (base.py:73-76, 119)
+        idx = (self.n_dof_per_node * self.elements).unsqueeze(-1) + torch.arange(
+            self.n_dof_per_node, device=device
+        )
+        self.idx = idx.reshape(self.n_elem, -1).to(torch.int32)
+
+        t0 = time.perf_counter()
+        self._pattern = DevicePattern(self.nodes, self.elements, self.n_dof_per_node,
+                                      self.etype.__name__, device)
+        self.setup_time = time.perf_counter() - t0
+
+        if material is None or material.is_vectorized:
+            self.material = material
+        else:
+            self.material = material.vectorize(self.n_elem)
+        self.cached_solve = CachedSolve()
+        self._pcg_handle: PCGSolver | None = None
+        self._last_system: _System | None = None
+        self._K_cache = None
+        self.last_solve_info: dict = {}
+
+    # ------------------------------------------------------------- abstract
+    @property
+    def n_state(self) -> int:
+        return 0 if self.material is None else self.material.n_state
+
+    @property
+    def volume_scale(self) -> Tensor:
+        return torch.ones(self.n_elem, dtype=torch.float64, device=self.device)
+
+    @property
+    @abstractmethod
+    def n_flux(self) -> list[int]: ...
+
+    @property
+    @abstractmethod
+    def n_dof_per_node(self) -> int: ...
+
+    @property
+    @abstractmethod
+    def etype(self) -> type[Element]: ...
+
+    @property
+    @abstractmethod
+    def initial_grad(self) -> Tensor: ...
+
+    def _scale_tensor(self) -> Tensor | None:
+        """Per-element volume scale passed to the kernels (thickness), or None."""
+        return None
+
+    # ----------------------------------------------------- reference indices
+    @cached_property
+    def glob_idx(self) -> Tensor:
+        """Row/column indices [2, nnz] int64 of the pattern (base.py:112-117)."""
+        e = self._pattern.expand(rows=True, cols=True)
+        return torch.stack([e["rows"], e["cols"]])
+
+    @cached_property
+    def k_map(self) -> Tensor:
+        """Element entry -> value slot, int32 [n_elem*(npe*ndof)^2] (base.py:94-104)."""
+        return self._pattern.expand(rows=False, cols=False, k_map=True)["k_map"]
+
+    @cached_property
+    def diag_map(self) -> Tensor:
+        """Diagonal slots, int32 [n_dofs] (base.py:106-108)."""
+        return self._pattern.expand(rows=False, cols=False, diag_map=True)["diag_map"]
+
+    # ------------------------------------------------------ boundary setters
+    @property
+    def constraints(self) -> Tensor:
+        return self._constraints
+
+    @constraints.setter
+    def constraints(self, value: Tensor):
+        if not value.shape == (self.n_nod, self.n_dof_per_node):
+            raise ValueError("Constraints must have the same shape as nodes.")
+        if value.dtype != torch.bool:
+            raise TypeError("Constraints must be a boolean tensor.")
+        self._constraints = value.to(self.device)
+
+    def _con_mask(self) -> Tensor:
+        return self._constraints.reshape(-1).to(torch.uint8).contiguous()
+
+    # ---------------------------------------------------- shape functions
+    def eval_shape_functions(self, xi: Tensor) -> tuple[Tensor, Tensor, Tensor]:
+        """N, B = J^{-1} dN/dxi and detJ at local points (base.py:293-314)."""
+        nodes = self.nodes[self.elements, :]
+        xi = torch.as_tensor(xi, dtype=torch.float64).to(self.device)
+        b = self.etype.B(xi).to(self.device)
+        J = torch.einsum("...iN, ANj -> ...Aij", b, nodes)
+        detJ = torch.linalg.det(J)
+        if torch.any(detJ <= 0.0):
+            raise ValueError("Negative Jacobian. Check element numbering.")
+        B = torch.einsum("...Eij,...jN->...EiN", torch.linalg.inv(J), b)
+        return self.etype.N(xi).to(self.device), B, detJ
+
+    def compute_B(self) -> Tensor:
+        """Rigid-body modes (base.py:316-344)."""
+        n, d = self.n_nod, self.n_dof_per_node
+        X = self.nodes
+        if d == 3:
+            B = torch.zeros((3 * n, 6), dtype=torch.float64, device=self.device)
+            B[0::3, 0] = 1
+            B[1::3, 1] = 1
+            B[2::3, 2] = 1
+            B[1::3, 3] = -X[:, 2]
+            B[2::3, 3] = X[:, 1]
+            B[0::3, 4] = X[:, 2]
+            B[2::3, 4] = -X[:, 0]
+            B[0::3, 5] = -X[:, 1]
+            B[1::3, 5] = X[:, 0]
+        elif d == 2:
+            B = torch.zeros((2 * n, 3), dtype=torch.float64, device=self.device)
+            B[0::2, 0] = 1
+            B[1::2, 1] = 1
+            B[1::2, 2] = -X[:, 0]
+            B[0::2, 2] = X[:, 1]
+        else:
+            B = torch.ones((n, 1), dtype=torch.float64, device=self.device)
+        return B
+
+    def integrate_shape_functions(self) -> Tensor:
+        N, _, detJ = self.eval_shape_functions(self.etype.ipoints)
+        w = self.etype.iweights.to(device=self.device, dtype=torch.float64)
+        return torch.einsum("i,in,ie->en", w, N, detJ)
+
+    def integrate_field(self, field: Tensor | None = None) -> Tensor:
+        w = self.integrate_shape_functions()
+        if field is None:
+            return w.sum(dim=1)
+        return (w * field[self.elements]).sum(dim=1)
+
+    # --------------------------------------------------------- kernels
+    def _check_jacobian(self, bad: Tensor) -> None:
+        if int(bad.item()) != 0:
+            raise ValueError("Negative Jacobian. Check element numbering.")
+
+    def _material_params(self):
+        if self.material is None:
+            raise ValueError("this model has no material")
+        kind, p0, p1 = self.material.kernel_params(self.n_elem, self.device)
+        return kind, p0, p1, self._scale_tensor()
+
+    def _forces(self, mat: L.Material, u: Tensor, means=False, per_gp=False,
+                flux_prev: Tensor | None = None):
+        """Internal forces K u (FP64, matrix-free) and optional gradients/fluxes."""
+        pat = self._pattern
+        dev = pat.device
+        u = u.reshape(-1).to(torch.float64).contiguous()
+        F = torch.empty(pat.n_dofs, dtype=torch.float64, device=dev)
+        work = torch.empty(pat.n_elem * pat.npe * pat.ndof, dtype=torch.float64, device=dev)
+        nq = pat.ndof * pat.dim
+        gm = torch.empty(pat.n_elem, nq, dtype=torch.float64, device=dev) if means else None
+        qm = torch.empty(pat.n_elem, nq, dtype=torch.float64, device=dev) if means else None
+        gg = torch.empty(self.n_int, pat.n_elem, nq, dtype=torch.float64, device=dev) if per_gp else None
+        qg = torch.empty(self.n_int, pat.n_elem, nq, dtype=torch.float64, device=dev) if per_gp else None
+        qp = flux_prev.reshape(self.n_int, pat.n_elem, nq).to(torch.float64).contiguous() if flux_prev is not None else None
+        L.check(L.lib().tfm_internal_force(pat.stream(), C.byref(pat.mesh), C.byref(mat),
+                                           L.ptr(u), L.ptr(qp), L.ptr(F), L.ptr(gm), L.ptr(qm),
+                                           L.ptr(gg), L.ptr(qg), L.ptr(work)))
+        return F, work, gm, qm, gg, qg
+
+    def _solver(self) -> PCGSolver:
+        if self._pcg_handle is None:
+            self._pcg_handle = PCGSolver(self.n_dofs, nrhs=1, device=self.device)
+        return self._pcg_handle
+
+    def _pcg(self, system: _System, b: Tensor, cfg: dict, warm: Tensor | None = None) -> Tensor:
+        """K_bc x = b for rows of b ([nrhs, n]) with the configured PCG."""
+        b = b.to(torch.float64).contiguous()
+        x = torch.zeros_like(b) if warm is None else warm.reshape(b.shape).to(torch.float64).clone()
+        solver = self._solver()
+        info = solver.solve_system(system.sys, b, x, precision=cfg["precision"],
+                                   preconditioner=cfg["pc"], rtol=cfg["stol"],
+                                   inner_rtol=cfg["inner_rtol"])
+        self._last_pcg = info
+        return x
+
+    def _newton(self, system: _System, cfg: dict, F_ext: Tensor, u0: Tensor, con: Tensor) -> Tensor:
+        """Newton loop of NewtonRaphsonAdjoint.forward (sparse.py:541-639) on the
+        correction d (zero on constrained DOFs) of a linear model."""
+        t0 = time.perf_counter()
+        zero = torch.zeros_like(F_ext)
+        r = torch.where(con, zero, F_ext - system.force(u0))
+        res0 = float(torch.linalg.norm(r))
+        d = torch.zeros_like(F_ext)
+        iters = []
+        for i in range(cfg["max_iter"]):
+            res = res0 if i == 0 else float(torch.linalg.norm(r))
+            if not math.isfinite(res):
+                break
+            if res < cfg["rtol"] * res0 or res < cfg["atol"]:
+                self.last_solve_info["forward"] = dict(
+                    newton_iterations=i, pcg=iters, res0=res0, res=res,
+                    time=time.perf_counter() - t0)
+                return d
+            warm = cfg.get("x0") if i == 0 else None
+            dd = self._pcg(system, r.unsqueeze(0), cfg, warm=warm)[0]
+            iters.append(self._last_pcg)
+            if i == 0 and cfg.get("cached") is not None and cfg.get("update_cache"):
+                cfg["cached"].update_x(dd)
+            d = d + dd
+            r = torch.where(con, zero, F_ext - system.force(u0 + d))
+        raise RuntimeError("Newton-Raphson iteration did not converge.")
+
+    # ------------------------------------------------------- API methods
+    def k0(self) -> Tensor:
+        """Element matrices of the reference state (base.py:228-245)."""
+        kind, p0, p1, scale = self._material_params()
+        h = _MatHolder(kind, p0, p1, scale)
+        pat = self._pattern
+        W = pat.npe * pat.ndof
+        k = torch.empty(pat.n_elem, W, W, dtype=torch.float64, device=self.device)
+        bad = torch.zeros(1, dtype=torch.int32, device=self.device)
+        L.check(L.lib().tfm_element_matrices(pat.stream(), C.byref(pat.mesh), C.byref(h.mat),
+                                             L.ptr(k), L.ptr(bad)))
+        self._check_jacobian(bad)
+        return k
+
+    def integrate_material(self, u_prev, grad_prev, flux_prev, state_prev, du, de0, iter, nlgeom,
+                           compute_stiffness: bool = True):
+        """Element stiffness, forces, gradients and fluxes for the increment du
+        from a previous state (base.py:247-278, 982-1092, 1177-1286)."""
+        if nlgeom:
+            raise NotImplementedError("nlgeom is not part of this build")
+        if de0 is not None and torch.as_tensor(de0).abs().max() > 0:
+            raise NotImplementedError("external gradient increments are not part of this build")
+        kind, p0, p1, scale = self._material_params()
+        h = _MatHolder(kind, p0, p1, scale)
+        _, fe, _, _, gg, qg = self._forces(h.mat, du, per_gp=True, flux_prev=flux_prev)
+        nd, dim = self.n_dof_per_node, self.n_dim
+        shape = (self.n_int, self.n_elem, *self.n_flux)
+        grad_new = grad_prev + gg.reshape(shape)
+        flux_new = qg.reshape(shape)
+        f = fe.reshape(self.n_elem, -1)
+        k = self.k0() if compute_stiffness else None
+        return k, f, grad_new, flux_new, state_prev.clone()
+
+    def assemble_matrix(self, k: Tensor, con: Tensor) -> Tensor:
+        """Global sparse matrix from element matrices with Dirichlet rows and
+        columns replaced by the identity (base.py:398-426)."""
+        pat = self._pattern
+        k = k.to(device=self.device, dtype=torch.float64).contiguous()
+        mask = torch.zeros(self.n_dofs, dtype=torch.uint8, device=self.device)
+        mask[torch.as_tensor(con, device=self.device).long()] = 1
+        val = torch.empty(pat.nnz, dtype=torch.float64, device=self.device)
+        diag = torch.empty(pat.n_nod * pat.ndof**2, dtype=torch.float64, device=self.device)
+        L.check(L.lib().tfm_assemble_elements(pat.stream(), C.byref(pat.mesh), L.ptr(k),
+                                              L.ptr(mask), L.ptr(val), None, L.ptr(diag)))
+        self.is_constrained = mask.bool()
+        with torch.sparse.check_sparse_tensor_invariants(False):
+            return torch.sparse_coo_tensor(self.glob_idx, val, size=(self.n_dofs, self.n_dofs),
+                                           is_coalesced=True)
+
+    def assemble_rhs(self, f: Tensor) -> Tensor:
+        """Global vector from element vectors [n_elem, npe*ndof] (base.py:428-445),
+        gathered in ascending element order."""
+        pat = self._pattern
+        if f.requires_grad:
+            # keep autograd: the gather is linear, its adjoint is an index select
+            return _Gather.apply(self, f)
+        fe = f.detach().to(device=self.device, dtype=torch.float64).contiguous()
+        F = torch.empty(self.n_dofs, dtype=torch.float64, device=self.device)
+        L.check(L.lib().tfm_gather_nodes(pat.stream(), C.byref(pat.mesh), L.ptr(fe), L.ptr(F)))
+        return F
+
+    @property
+    def K(self) -> Tensor:
+        """The last assembled global stiffness as sparse COO (base.py:699, 738)."""
+        if self._K_cache is not None:
+            return self._K_cache
+        s = self._last_system
+        if s is None:
+            return torch.empty(0, device=self.device)
+        if s.val64 is not None:
+            val = s.val64
+        else:  # mixed solves keep FP32 values only: assemble FP64 on demand
+            s64 = _System(self, s.kind, s.p0, s.p1, s.scale, "fp64")
+            val = s64.val64
+        with torch.sparse.check_sparse_tensor_invariants(False):
+            self._K_cache = torch.sparse_coo_tensor(self.glob_idx, val,
+                                                    size=(self.n_dofs, self.n_dofs),
+                                                    is_coalesced=True)
+        return self._K_cache
+
+    @K.setter
+    def K(self, value):
+        self._K_cache = None if (value is None or value.numel() == 0) else value
+
+    # -------------------------------------------------------------- loads
+    def _scatter(self, conn: Tensor, contrib: Tensor) -> Tensor:
+        f = torch.zeros(self.n_nod, contrib.shape[-1], dtype=torch.float64, device=self.device)
+        return f.index_add_(0, conn.ravel(), contrib.flatten(0, 1))
+
+    def _boundary_facets(self, mask: Tensor) -> Tensor:
+        """Boundary facets with all nodes in `mask` (base.py:452-470)."""
+        table = self.etype.facets.to(self.device)
+        facets = self.elements[:, table].reshape(-1, table.shape[1])
+        facets = facets[mask.to(self.device)[facets].all(dim=1)]
+        _, inv, count = torch.unique(facets.sort(dim=1).values, dim=0, return_inverse=True,
+                                     return_counts=True)
+        first = torch.full((len(count),), len(inv), device=self.device)
+        first.scatter_reduce_(0, inv, torch.arange(len(inv), device=self.device), reduce="amin")
+        return facets[first[count == 1]]
+
+    def _integrate_facet_load(self, conn: Tensor, ftype, load: Tensor) -> Tensor:
+        """Consistent nodal loads on facets (base.py:472-494)."""
+        xi = ftype.ipoints.to(self.device)
+        J = torch.einsum("iaN,eNj->ieaj", ftype.B(xi).to(self.device), self.nodes[conn])
+        detJ = torch.sqrt(torch.linalg.det(J @ J.transpose(-1, -2)))
+        load = load.to(device=self.device, dtype=torch.float64)
+        if load.dim() == 0 and self.n_dof_per_node > 1:
+            if J.shape[-2] == 2:
+                n = torch.linalg.cross(J[..., 0, :], J[..., 1, :], dim=-1)
+            else:
+                n = torch.stack([J[..., 0, 1], -J[..., 0, 0]], dim=-1)
+            val = load * torch.nn.functional.normalize(n, dim=-1)
+        else:
+            per_facet = load if load.dim() == 2 else load.reshape(1, -1)
+            val = per_facet.expand(len(conn), -1).expand(len(xi), -1, -1)
+        w = ftype.iweights.to(device=self.device, dtype=torch.float64)
+        contrib = torch.einsum("i,in,ie,iek->enk", w, ftype.N(xi).to(self.device), detJ, val)
+        return self._scatter(conn, contrib)
+
+    def integrate_body_load(self, load) -> Tensor:
+        """Consistent nodal loads from a load per unit volume (base.py:496-513)."""
+        load = torch.as_tensor(load, dtype=torch.float64, device=self.device)
+        per_elem = load if load.dim() == 2 else load.reshape(1, -1)
+        w = self.integrate_shape_functions() * self.volume_scale[:, None]
+        contrib = torch.einsum("en,ek->enk", w, per_elem.expand(self.n_elem, -1))
+        return self._scatter(self.elements, contrib)
+
+    def integrate_surface_load(self, mask: Tensor, load) -> Tensor:
+        """Consistent nodal loads from a load per unit area (base.py:515-539)."""
+        if self.etype.iso_dim != 3:
+            raise NotImplementedError(
+                f"{type(self).__name__} has no surfaces to load. Use "
+                "integrate_line_load(...) or integrate_body_load(...) instead."
+            )
+        return self._integrate_facet_load(self._boundary_facets(mask), self.etype.facet_type,
+                                          torch.as_tensor(load, dtype=torch.float64))
+
+    def integrate_line_load(self, mask: Tensor, load) -> Tensor:
+        """Consistent nodal loads from a load per unit length (base.py:541-568)."""
+        if self.etype.iso_dim != 2:
+            raise NotImplementedError(
+                f"{type(self).__name__} has no edges to load. Use "
+                "integrate_surface_load(...) or integrate_body_load(...) instead."
+            )
+        load = torch.as_tensor(load, dtype=torch.float64)
+        if load.dim() == 0 and self.n_dim == 3:
+            raise ValueError("A line in 3D has no unique normal, so a scalar load is ambiguous. "
+                             "Pass a load vector instead.")
+        return self._integrate_facet_load(self._boundary_facets(mask), self.etype.facet_type, load)
+
+    # --------------------------------------------------------------- solve
+    def solve(
+        self,
+        increments: Tensor | None = None,
+        max_iter: int = 10,
+        rtol: float = 1e-8,
+        atol: float = 1e-6,
+        stol: float = 1e-10,
+        cutback_factor: float = 0.5,
+        growth_factor: float = 1.1,
+        max_cutbacks: int = 10,
+        verbose: bool = False,
+        method: str | None = None,
+        device: str | None = None,
+        return_intermediate: bool = False,
+        aggregate_integration_points: bool = True,
+        use_cached_solve: bool = False,
+        nlgeom: bool = False,
+        alpha: float = 0.0,
+        differentiable_parameters: Tensor | Iterable[Tensor] | None = None,
+        precision: str | None = None,
+        preconditioner: str = "block_jacobi",
+        inner_rtol: float = 1e-5,
+    ):
+        """Quasi-static solve by load increments (base.py:597-923).
+
+        Extra keywords of this build: `precision` ("mixed": FP32-inner PCG with
+        FP64 refinement, the default; "fp64": all-FP64 PCG), `preconditioner`
+        ("block_jacobi" or "jacobi") and `inner_rtol` (mixed-mode inner tolerance).
+        """
+        if nlgeom:
+            raise NotImplementedError("nlgeom (geometric nonlinearity) is not part of this build")
+        if alpha > 0.0:
+            raise NotImplementedError("viscous stabilization is not part of this build")
+        if self.n_state != 0:
+            raise NotImplementedError("stateful materials are not part of this build")
+        if self._external_gradient.abs().max() > 0:
+            raise NotImplementedError("external strains are not part of this build")
+        if method not in (None, "cg", "minres"):
+            if method in ("spsolve", "pardiso", "amgx"):
+                raise RuntimeError(
+                    f"method {method!r} is not available in this build: FEM solves run on the "
+                    "hand-written PCG (method None, 'cg' or 'minres')"
+                )
+            raise ValueError(
+                f"Method {method} is not supported. "
+                "Choose from 'spsolve', 'minres', 'cg', 'pardiso', or 'amgx'."
+            )
+        precision = precision or default_precision()
+        if precision not in _PREC:
+            raise ValueError(f"precision must be one of {sorted(_PREC)}")
+        if preconditioner not in _PC:
+            raise ValueError(f"preconditioner must be one of {sorted(_PC)}")
+
+        increments = torch.tensor([0.0, 1.0]) if increments is None else torch.as_tensor(increments)
+        incs = [float(v) for v in increments]
+        N = len(incs)
+
+        if differentiable_parameters is None:
+            params = ()
+        elif isinstance(differentiable_parameters, Tensor):
+            params = (differentiable_parameters,)
+        else:
+            params = tuple(differentiable_parameters)
+        track = any(p.requires_grad for p in params)
+
+        kind, p0, p1, scale = self._material_params()
+        cfg = dict(kind=kind, precision=precision, pc=preconditioner, stol=stol, rtol=rtol,
+                   atol=atol, max_iter=max_iter, inner_rtol=inner_rtol)
+        cached = self.cached_solve if use_cached_solve else None
+        if cached is not None:
+            cfg.update(cached=cached, update_cache=True, x0=cached.previous_x,
+                       x0_adjoint=cached.previous_grad)
+        self.last_solve_info = {}
+        self._K_cache = None
+
+        lam0 = incs[0]
+        F_unit = self._neumann.reshape(-1)
+        D_unit = self._dirichlet.reshape(-1)
+        if not track:
+            F_unit, D_unit = F_unit.detach(), D_unit.detach()
+            p0 = p0.detach()
+            p1 = p1.detach() if p1 is not None else None
+            scale = scale.detach() if scale is not None else None
+
+        # unit-load solutions: forces scale with the load factor, prescribed
+        # values with its change since the first increment (base.py:761-770)
+        with torch.set_grad_enabled(track):
+            split = lam0 != 0.0 and bool(D_unit.abs().max() > 0) and bool(F_unit.abs().max() > 0)
+            if split:
+                uF = _StaticSolve.apply(self, cfg, F_unit, torch.zeros_like(D_unit), p0, p1, scale)
+                uD = _StaticSolve.apply(self, cfg, torch.zeros_like(F_unit), D_unit, p0, p1, scale)
+            else:
+                uF = _StaticSolve.apply(self, cfg, F_unit, D_unit, p0, p1, scale)
+                uD = None
+            fF, gF, qF = _Integrate.apply(self, kind, uF, p0, p1, scale)
+            if uD is not None:
+                fD, gD, qD = _Integrate.apply(self, kind, uD, p0, p1, scale)
+
+            nd = self.n_dof_per_node
+            flux_shape = (self.n_elem, *self.n_flux)
+            u_out, f_out, q_out, g_out = [], [], [], []
+            init = self.initial_grad.to(self.device, torch.float64)
+            d_only = (not split) and lam0 != 0.0 and bool(D_unit.abs().max() > 0)
+            for n in range(N):
+                # forces are totals of the load factor, prescribed values grow
+                # with its change since the first increment
+                a = 0.0 if n == 0 else (incs[n] - lam0 if d_only else incs[n])
+                b = 0.0 if n == 0 else incs[n] - lam0
+                u = a * uF
+                f = a * fF
+                g = a * gF
+                q = a * qF
+                if uD is not None:
+                    u = u + b * uD
+                    f = f + b * fD
+                    g = g + b * gD
+                    q = q + b * qD
+                u_out.append(u.reshape(-1, nd))
+                f_out.append(f.reshape(-1, nd))
+                g_out.append(init + g.reshape(flux_shape))
+                q_out.append(q.reshape(flux_shape))
+
+        u_all = torch.stack(u_out)
+        f_all = torch.stack(f_out)
+        q_all = torch.stack(q_out)
+        g_all = torch.stack(g_out)
+        s_all = torch.zeros(N, self.n_elem, self.n_state, dtype=torch.float64, device=self.device)
+        if not aggregate_integration_points:
+            # a linear model from a zero state has the same flux at every
+            # Gauss point only for constant-strain elements; report per point
+            q_all, g_all = self._per_gp_outputs(u_all, init)
+            s_all = torch.zeros(N, self.n_int, self.n_elem, self.n_state, dtype=torch.float64,
+                                device=self.device)
+        q_all = q_all.squeeze((-2, -1)) if q_all.dim() > 2 else q_all
+        g_all = g_all.squeeze((-2, -1)) if g_all.dim() > 2 else g_all
+        if not track:
+            u_all, f_all, q_all, g_all = (t.detach() for t in (u_all, f_all, q_all, g_all))
+        if verbose:
+            self._print_report(precision, preconditioner)
+        if return_intermediate:
+            return u_all, f_all, q_all, g_all, s_all
+        return u_all[-1], f_all[-1], q_all[-1], g_all[-1], s_all[-1]
+
+    def _per_gp_outputs(self, u_all: Tensor, init: Tensor):
+        kind, p0, p1, scale = self._material_params()
+        h = _MatHolder(kind, p0.detach(), p1.detach() if p1 is not None else None,
+                       scale.detach() if scale is not None else None)
+        qs, gs = [], []
+        shape = (self.n_int, self.n_elem, *self.n_flux)
+        for u in u_all:
+            _, _, _, _, gg, qg = self._forces(h.mat, u.detach(), per_gp=True)
+            gs.append(init + gg.reshape(shape))
+            qs.append(qg.reshape(shape))
+        return torch.stack(qs), torch.stack(gs)
+
+    def _print_report(self, precision, pc):
+        info = self.last_solve_info.get("forward", {})
+        its = [p.get("iterations") for p in info.get("pcg", [])]
+        print(f"torch-fem (MI355X) | {type(self).__name__} | {self.n_elem:,} elem | "
+              f"{self.n_dofs:,} dof | {precision} PCG ({pc}) | newton {info.get('newton_iterations')}"
+              f" | pcg iterations {its} | res {info.get('res', float('nan')):.3e}")
+
+
+class _Gather(torch.autograd.Function):
+    """assemble_rhs with autograd: F = sum over elements; dF/df = index select."""
+
+    @staticmethod
+    def forward(ctx, fem, f):
+        ctx.fem = fem
+        return fem.assemble_rhs(f.detach())
+
+    @staticmethod
+    def backward(ctx, g):
+        fem = ctx.fem
+        return None, g[fem.idx.long()]
+
+
+class Mechanics(FEM, ABC):
+    """Solid and structural mechanics (base.py:926-1126)."""
+
+    @property
+    def n_dof_per_node(self) -> int:
+        return self.nodes.shape[1]
+
+    @property
+    def initial_grad(self) -> Tensor:
+        return torch.eye(self.n_flux[0], dtype=torch.float64, device=self.device)
+
+    @property
+    def forces(self) -> Tensor:
+        return self._neumann
+
+    @forces.setter
+    def forces(self, value: Tensor):
+        if not value.shape == (self.n_nod, self.n_dof_per_node):
+            raise ValueError("Forces must have the same shape as nodes.")
+        if not torch.is_floating_point(value):
+            raise TypeError("Forces must be a floating-point tensor.")
+        self._neumann = value.to(self.device)
+
+    @property
+    def displacements(self) -> Tensor:
+        return self._dirichlet
+
+    @displacements.setter
+    def displacements(self, value: Tensor):
+        if not value.shape == (self.n_nod, self.n_dof_per_node):
+            raise ValueError("Displacements must have the same shape as nodes.")
+        if not torch.is_floating_point(value):
+            raise TypeError("Displacements must be a floating-point tensor.")
+        self._dirichlet = value.to(self.device)
+
+    @property
+    def ext_strain(self) -> Tensor:
+        return self._external_gradient
+
+    @ext_strain.setter
+    def ext_strain(self, value: Tensor):
+        if not value.shape == (self.n_elem, self.n_dof_per_node, self.n_dim):
+            raise ValueError("External strain must have the same shape as strains.")
+        if not torch.is_floating_point(value):
+            raise TypeError("External strain must be a floating-point tensor.")
+        self._external_gradient = value.to(self.device)
+
+
+class Heat(FEM, ABC):
+    """Steady heat conduction (base.py:1129-1286)."""
+
+    @property
+    def n_dof_per_node(self) -> int:
+        return 1
+
+    @property
+    def n_flux(self) -> list[int]:
+        return [1, self.n_dim]
+
+    @property
+    def initial_grad(self) -> Tensor:
+        return torch.zeros(1, dtype=torch.float64, device=self.device)
+
+    @property
+    def heat_flux(self) -> Tensor:
+        return self._neumann
+
+    @heat_flux.setter
+    def heat_flux(self, value: Tensor):
+        if not value.shape == (self.n_nod, 1):
+            raise ValueError("Heat flux must have the same shape as nodes.")
+        if not torch.is_floating_point(value):
+            raise TypeError("Heat flux must be a floating-point tensor.")
+        self._neumann = value.to(self.device)
+
+    @property
+    def temperatures(self) -> Tensor:
+        return self._dirichlet
+
+    @temperatures.setter
+    def temperatures(self, value: Tensor):
+        if not value.shape == (self.n_nod, self.n_dof_per_node):
+            raise ValueError("Temperatures must have the same shape as nodes.")
+        if not torch.is_floating_point(value):
+            raise TypeError("Temperatures must be a floating-point tensor.")
+        self._dirichlet = value.to(self.device)

@@ -1,0 +1,187 @@
+"""Linear, stateless materials of the assemble-and-solve path.
+
+Same classes, attributes and `step` contract as the reference
+(src/torchfem/materials/base.py:29-91, elasticity.py:12-182,
+conductivity.py:11-98). Each material also tells the HIP kernels how to read
+its parameters: isotropic materials pass two (or one) numbers per element
+(lam, G or kappa) instead of a materialised 81-entry tangent; assigning a
+tangent (`material.C = ...` or `material.KAPPA = ...`, e.g. a SIMP-scaled C)
+switches to the general layout, which reads the full tensor per element.
+Autograd links from these parameters to user design variables are kept, so
+the adjoint gradients flow back to them.
+"""
+
+from __future__ import annotations
+
+import copy
+from abc import ABC, abstractmethod
+
+import torch
+from torch import Tensor
+
+from .. import _lib as L
+
+__all__ = [
+    "Material",
+    "IsotropicElasticity3D",
+    "IsotropicElasticityPlaneStress",
+    "IsotropicConductivity3D",
+    "IsotropicConductivity2D",
+]
+
+
+def _t(x) -> Tensor:
+    return torch.as_tensor(x, dtype=torch.get_default_dtype()) if not isinstance(x, Tensor) else x
+
+
+class Material(ABC):
+    """Base class (materials/base.py:10-103)."""
+
+    n_state: int = 0
+    is_vectorized: bool = False
+
+    def __init__(self):
+        self.n_state = 0
+        self.is_vectorized = False
+        self.rho = torch.tensor(1.0)
+
+    def vectorize(self, n_elem: int) -> "Material":
+        """Returns the material batched over `n_elem` elements (base.py:29-46)."""
+        if self.is_vectorized:
+            return self
+        new = copy.copy(self)
+        for key, value in list(vars(new).items()):
+            if isinstance(value, Tensor):
+                setattr(new, key, value.repeat(n_elem, *(value.dim() * [1])))
+        new.is_vectorized = True
+        return new
+
+    @abstractmethod
+    def step(self, H_inc, F, stress, state, de0, cl, iter):
+        """Incremental update (base.py:48-91): (stress, state, tangent)."""
+
+    def rotate(self, R: Tensor) -> "Material":
+        return self
+
+    # ------------------------------------------------------------ kernels
+    @abstractmethod
+    def kernel_params(self, n_elem: int, device) -> tuple[int, Tensor, Tensor | None]:
+        """(layout, p0, p1) for libtfm_hip: FP64 per-element tensors on `device`."""
+
+
+def _per_element(x: Tensor, n_elem: int, device) -> Tensor:
+    x = torch.as_tensor(x).to(device=device, dtype=torch.float64)
+    return torch.broadcast_to(x.reshape(-1) if x.dim() else x, (n_elem,)).contiguous()
+
+
+class IsotropicElasticity3D(Material):
+    """Isotropic linear elasticity in 3D (elasticity.py:12-127).
+
+    lam = E nu / ((1 + nu)(1 - 2 nu)), G = E / (2 (1 + nu)),
+    C_ijkl = lam d_ij d_kl + G (d_ik d_jl + d_il d_jk).
+    """
+
+    dim = 3
+
+    def __init__(self, E: Tensor | float, nu: Tensor | float, rho: Tensor | float = 1.0):
+        self.E = _t(E)
+        self.nu = _t(nu)
+        self.rho = _t(rho)
+        self.n_state = 0
+        self.is_vectorized = self.E.dim() > 0
+        self.lbd = self.E * self.nu / ((1.0 + self.nu) * (1.0 - 2.0 * self.nu))
+        self.G = self.E / (2.0 * (1.0 + self.nu))
+        self._C: Tensor | None = None
+
+    def _iso_lambda(self) -> Tensor:
+        return self.lbd
+
+    @property
+    def C(self) -> Tensor:
+        """Stiffness tensor [..., d, d, d, d]; assigned values override (lam, G)."""
+        if self._C is not None:
+            return self._C
+        d = self.dim
+        I2 = torch.eye(d, dtype=self.G.dtype, device=self.G.device)
+        I4 = torch.einsum("ij,kl->ijkl", I2, I2)
+        I4S = torch.einsum("ik,jl->ijkl", I2, I2) + torch.einsum("il,jk->ijkl", I2, I2)
+        lam = self._iso_lambda()[..., None, None, None, None]
+        return lam * I4 + self.G[..., None, None, None, None] * I4S
+
+    @C.setter
+    def C(self, value: Tensor) -> None:
+        self._C = value
+
+    def step(self, H_inc, F, stress, state, de0, cl, iter):
+        """sigma_{n+1} = sigma_n + C : (sym(H_inc) - de0), tangent C (elasticity.py:119-127)."""
+        de = 0.5 * (H_inc.transpose(-1, -2) + H_inc)
+        C = self.C
+        stress_new = stress + torch.einsum("...ijkl,...kl->...ij", C, de - de0)
+        return stress_new, state, C
+
+    def kernel_params(self, n_elem, device):
+        if self._C is None:
+            return (
+                L.MAT_ISO,
+                _per_element(self._iso_lambda(), n_elem, device),
+                _per_element(self.G, n_elem, device),
+            )
+        d = self.dim
+        C = torch.as_tensor(self._C).to(device=device, dtype=torch.float64)
+        C = torch.broadcast_to(C, (n_elem, d, d, d, d)).reshape(n_elem, d**4).contiguous()
+        return L.MAT_GENERAL, C, None
+
+
+class IsotropicElasticityPlaneStress(IsotropicElasticity3D):
+    """Plane stress (elasticity.py:130-182): C0000 = C1111 = E/(1-nu^2),
+    C0011 = C1100 = E nu/(1-nu^2), C0101 = C0110 = C1001 = C1010 = E/(2(1+nu)),
+    i.e. the isotropic form with lam' = E nu / (1 - nu^2)."""
+
+    dim = 2
+
+    def _iso_lambda(self) -> Tensor:
+        return self.E * self.nu / (1.0 - self.nu**2)
+
+
+class IsotropicConductivity3D(Material):
+    """Isotropic heat conduction (conductivity.py:11-98): KAPPA = kappa I,
+    flux q = q_n + KAPPA (grad T_inc - de0)."""
+
+    dim = 3
+
+    def __init__(self, kappa: Tensor | float, rho: Tensor | float = 1.0):
+        self.kappa = _t(kappa)
+        self.rho = _t(rho)
+        self.n_state = 0
+        self.is_vectorized = self.kappa.dim() > 0
+        self._KAPPA: Tensor | None = None
+
+    @property
+    def KAPPA(self) -> Tensor:
+        if self._KAPPA is not None:
+            return self._KAPPA
+        return self.kappa[..., None, None] * torch.eye(self.dim, dtype=self.kappa.dtype,
+                                                       device=self.kappa.device)
+
+    @KAPPA.setter
+    def KAPPA(self, value: Tensor) -> None:
+        self._KAPPA = value
+
+    def step(self, H_inc, F, stress, state, de0, cl, iter):
+        K = self.KAPPA
+        q = stress + torch.einsum("...ij,...kj->...ki", K, H_inc - de0)
+        return q, state, K
+
+    def kernel_params(self, n_elem, device):
+        if self._KAPPA is None:
+            return L.MAT_HEAT_ISO, _per_element(self.kappa, n_elem, device), None
+        d = self.dim
+        K = torch.as_tensor(self._KAPPA).to(device=device, dtype=torch.float64)
+        K = torch.broadcast_to(K, (n_elem, d, d)).reshape(n_elem, d * d).contiguous()
+        return L.MAT_HEAT_GENERAL, K, None
+
+
+class IsotropicConductivity2D(IsotropicConductivity3D):
+    """Planar isotropic heat conduction (conductivity.py, 2D variant)."""
+
+    dim = 2

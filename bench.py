@@ -1,0 +1,252 @@
+"""Benchmark: assembly + solve + adjoint gradient on the Hexa1 64^3 cube.
+
+Workload (one step): a Hexa1 cube on a structured 64^3 mesh over [0,1]^3
+(274,625 nodes, 823,875 DOFs), isotropic elasticity E=1000, nu=0.3 with a
+per-element modulus tensor, z=0 clamped, u_z=-0.01 prescribed on z=1, 2x2x2
+Gauss points. One step is FEM.solve() (fused integration + assembly, PCG
+solve to 1e-8 relative residual) followed by the adjoint gradient of sum(u)
+with respect to the per-element moduli and the nodal forces (the published
+cube benchmark differentiates sum(u) w.r.t. forces). The pattern setup is
+timed separately, as in the reference's benchmark protocol.
+
+Prints one JSON line (stdout); progress goes to stderr.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT, ROOT / "torch-fem_amd"):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+METRIC = "solve+adjoint-grad time & DOFs/s, Hexa1 cube, mixed-prec vs FP64 @1e-8 rel"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse() -> argparse.Namespace:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=64, help="cells per cube edge")
+    ap.add_argument("--stol", type=float, default=1e-8)
+    ap.add_argument("--preconditioner", default="block_jacobi")
+    ap.add_argument("--inner-rtol", type=float, default=0.1,
+                    help="mixed precision: residual drop between reliable updates")
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--cpu-n", type=int, default=32, help="cube size of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def build_model(n: int):
+    import torch
+
+    from torchfem import Solid
+    from torchfem.materials import IsotropicElasticity3D
+    from torchfem.mesh import cube_hexa
+
+    nodes, elements = cube_hexa(n + 1, n + 1, n + 1)
+    t0 = time.perf_counter()
+    model = Solid(nodes.cuda(), elements.cuda(), IsotropicElasticity3D(1000.0, 0.3))
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    X = model.nodes
+    model.constraints[X[:, 2] == 0.0, :] = True
+    model.constraints[X[:, 2] == 1.0, 2] = True
+    model.displacements[X[:, 2] == 1.0, 2] = -0.01
+    return model, setup
+
+
+def make_step(model, precision: str, stol: float, pc: str, inner_rtol: float):
+    import torch
+
+    from torchfem.materials import IsotropicElasticity3D
+
+    E0 = torch.full((model.n_elem,), 1000.0, dtype=torch.float64, device=model.device)
+
+    def step():
+        E = E0.clone().requires_grad_(True)
+        F = torch.zeros(model.n_nod, 3, dtype=torch.float64, device=model.device, requires_grad=True)
+        model.material = IsotropicElasticity3D(E, 0.3)
+        model.forces = F
+        u, f, sigma, _, _ = model.solve(differentiable_parameters=(E, F), precision=precision,
+                                        stol=stol, preconditioner=pc, inner_rtol=inner_rtol)
+        gE, gF = torch.autograd.grad(u.sum(), (E, F))
+        return u.detach(), gE, gF
+
+    return step
+
+
+def timed(step, steps: int, warmup: int, label: str):
+    import torch
+
+    for i in range(warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"{label}: warmup {i + 1}/{warmup} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for i in range(steps):
+        out = step()
+        log(f"{label}: step {i + 1}/{steps} done")
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, out
+
+
+def cpu_baseline(n: int) -> dict:
+    """The CPU oracle (FP64, torch einsum integration, index_add_ assembly,
+    SciPy Jacobi-CG at rtol 1e-10, autograd implicit adjoint) on a smaller cube
+    with the same boundary conditions and objective."""
+    import torch
+
+    import oracle as O
+
+    threads = min(os.cpu_count() or 1, 16)
+    torch.set_num_threads(threads)
+    nodes, el = O.cube_hexa(n + 1, n + 1, n + 1)
+    E = torch.full((len(el),), 1000.0, dtype=torch.float64, requires_grad=True)
+    F = torch.zeros(len(nodes), 3, dtype=torch.float64, requires_grad=True)
+    m = O.OracleModel(nodes, el, O.Hexa1, "solid", O.iso3d_C(E, torch.tensor(0.3, dtype=torch.float64)))
+    con = torch.zeros(len(nodes), 3, dtype=torch.bool)
+    con[nodes[:, 2] == 0.0, :] = True
+    con[nodes[:, 2] == 1.0, 2] = True
+    D = torch.zeros(len(nodes), 3, dtype=torch.float64)
+    D[nodes[:, 2] == 1.0, 2] = -0.01
+    t0 = time.perf_counter()
+    u, _, _, _ = m.solve(con, F, D, params=(E, F), rtol=1e-10)
+    t1 = time.perf_counter()
+    torch.autograd.grad(u.sum(), (E, F))
+    t2 = time.perf_counter()
+    dofs = 3 * len(nodes)
+    return {
+        "value": dofs / (t2 - t0),
+        "unit": "DOFs/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"Hexa1 cube {n}^3 cells ({dofs} DOFs), same BCs and objective; FP64; "
+                  f"SciPy Jacobi-CG rtol 1e-10 (pyamg absent); fwd {t1 - t0:.2f}s bwd {t2 - t1:.2f}s",
+    }
+
+
+def main() -> None:
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is "
+              "described in DESIGN.md but not built", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    log(f"building Hexa1 cube {args.n}^3")
+    model, setup = build_model(args.n)
+    pat = model._pattern
+    log(f"setup {setup:.2f}s | {model.n_dofs} DOFs | nnz {pat.nnz}")
+
+    step_mixed = make_step(model, "mixed", args.stol, args.preconditioner, args.inner_rtol)
+    solver = None
+    # warmup, then the timed steps with SpMV event timing on the solve stream
+    for i in range(args.warmup):
+        step_mixed()
+        torch.cuda.synchronize()
+        log(f"mixed: warmup {i + 1}/{args.warmup} done")
+    solver = model._solver()
+    solver.timing(True)
+    ms_mixed, (u_mix, gE_mix, gF_mix) = timed(step_mixed, args.steps, 0, "mixed")
+    spmv_ms, spmv_n = solver.timing(False)
+    info_mixed = dict(model.last_solve_info)
+
+    step_fp64 = make_step(model, "fp64", args.stol, args.preconditioner, args.inner_rtol)
+    ms_fp64, _ = timed(step_fp64, args.fp64_steps, 1, "fp64")
+    info_fp64 = dict(model.last_solve_info)
+
+    # FP64 reference of the same problem solved tightly, for the error report
+    step_ref = make_step(model, "fp64", 1e-11, args.preconditioner, args.inner_rtol)
+    u_ref, gE_ref, gF_ref = step_ref()
+    rel = lambda a, b: float((a - b).norm() / b.norm())  # noqa: E731
+    acc = {
+        "u_rel_err_vs_fp64": rel(u_mix, u_ref),
+        "gradE_rel_err_vs_fp64": rel(gE_mix, gE_ref),
+        "gradF_rel_err_vs_fp64": rel(gF_mix, gF_ref),
+        "relres_forward": info_mixed["forward"]["pcg"][-1]["relres"][0],
+        "relres_adjoint": info_mixed["adjoint"]["relres"][0],
+        "reference": "all-FP64 GPU PCG to 1e-11, whose parity with the CPU oracle is tested",
+    }
+    log(f"accuracy {acc}")
+
+    # roofline of the dominant kernel: FP32 node-block SpMV
+    n_adj = int(pat.adj.numel())
+    bytes_launch = pat.nnz * 4 + n_adj * 4 + (pat.n_nod + 1) * (4 + 8) + model.n_dofs * 4 * 2
+    avg_s = spmv_ms / max(spmv_n, 1) / 1e3
+    achieved = bytes_launch / avg_s / 1e9 if spmv_n else None
+    roof = {
+        "kernel": "k_spmv_nodes<float,3,16,1>",
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS if achieved else None,
+        "traffic": None,
+        "avg_launch_us": avg_s * 1e6,
+        "launches": spmv_n,
+        "algorithmic_bytes": bytes_launch,
+    }
+    it_fwd = sum(p["iterations"][0] for p in info_mixed["forward"]["pcg"])
+    it_adj = info_mixed["adjoint"]["iterations"][0]
+    it64_fwd = sum(p["iterations"][0] for p in info_fp64["forward"]["pcg"])
+    it64_adj = info_fp64["adjoint"]["iterations"][0]
+    result = {
+        "metric": METRIC,
+        "value": model.n_dofs / ms_mixed,
+        "unit": "DOFs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_mixed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Hexa1 cube {args.n}^3 (z=0 clamped, u_z=-0.01 on z=1), assembly + "
+                        "mixed-precision PCG to 1e-8 + adjoint d(sum u)/d(E_e, F)",
+            "n_dofs": model.n_dofs,
+            "nnz": pat.nnz,
+            "precision": "mixed (FP32 inner PCG, FP64 refinement)",
+            "preconditioner": args.preconditioner,
+            "stol": args.stol,
+        },
+        "setup_s": setup,
+        "fp64_ms_per_step": ms_fp64 * 1e3,
+        "speedup_vs_fp64": ms_fp64 / ms_mixed,
+        "pcg_iterations": {"mixed_fwd": it_fwd, "mixed_adj": it_adj,
+                           "mixed_outer_fwd": info_mixed["forward"]["pcg"][-1]["outer"],
+                           "mixed_outer_adj": info_mixed["adjoint"]["outer"],
+                           "fp64_fwd": it64_fwd, "fp64_adj": it64_adj},
+        "phase_s": {"mixed_fwd_pcg": info_mixed["forward"]["pcg"][-1]["time"],
+                    "mixed_adj_pcg": info_mixed["adjoint"]["time"]},
+        "accuracy": acc,
+        "roofline": roof,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        log(f"CPU baseline sample: Hexa1 {args.cpu_n}^3 with the oracle")
+        result["cpu_baseline"] = cpu_baseline(args.cpu_n)
+        log(f"CPU baseline {result['cpu_baseline']}")
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -235,6 +235,13 @@ int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t s, const tfm_system_t *sys,
                   int32_t nrhs, double rtol, double inner_rtol, int32_t *iters,
                   double *relres, int32_t *outer);
 
+/* Event timing of the inner SpMV launches of tfm_pcg_solve on the solve's
+ * stream: returns the accumulated milliseconds and launch count since the
+ * previous call, resets them, and enables (enable != 0) or disables timing.
+ * Measurement plumbing with no counterpart in the reference. */
+int tfm_pcg_timing(tfm_pcg_t *h, int32_t enable, double *spmv_ms,
+                   int64_t *spmv_launches);
+
 /* Preconditioned CG for a general SPD CSR matrix (FP64, Jacobi). */
 int tfm_csr_pcg_solve(tfm_pcg_t *h, tfm_stream_t s, int64_t n,
                       const int64_t *rowptr, const int32_t *col,

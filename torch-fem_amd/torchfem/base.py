@@ -608,13 +608,14 @@ class FEM(ABC):
         differentiable_parameters: Tensor | Iterable[Tensor] | None = None,
         precision: str | None = None,
         preconditioner: str = "block_jacobi",
-        inner_rtol: float = 1e-5,
+        inner_rtol: float = 0.1,
     ):
         """Quasi-static solve by load increments (base.py:597-923).
 
         Extra keywords of this build: `precision` ("mixed": FP32-inner PCG with
         FP64 refinement, the default; "fp64": all-FP64 PCG), `preconditioner`
-        ("block_jacobi" or "jacobi") and `inner_rtol` (mixed-mode inner tolerance).
+        ("block_jacobi" or "jacobi") and `inner_rtol` (mixed precision: the factor by
+        which the FP32 recursive residual drops between two reliable updates).
         """
         if nlgeom:
             raise NotImplementedError("nlgeom (geometric nonlinearity) is not part of this build")

@@ -270,3 +270,80 @@ def test_negative_jacobian_raises():
     model.constraints[0] = True
     with pytest.raises(ValueError, match="Negative Jacobian"):
         model.solve()
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_batched_load_cases_hexa2(precision):
+    """Config 5 reduced to 3^3 Hexa2 with 5 traction load cases (padded to 8
+    interleaved right-hand sides): every case against the oracle, and the
+    gradient of the summed compliances against the sum of oracle gradients."""
+    nodes, el = O.linear_to_quadratic(*O.cube_hexa(4, 4, 4))
+    E0 = 1000.0 * torch.ones(len(el), dtype=DT)
+    Eg = E0.clone().cuda().requires_grad_(True)
+    model = Solid(nodes.cuda(), el.cuda(), IsotropicElasticity3D(Eg, 0.3))
+    X = model.nodes
+    model.constraints[X[:, 2] == 0.0] = True
+    top = X[:, 2] == 1.0
+    Er = E0.clone().requires_grad_(True)
+    loads, oracle_u = [], []
+    go = torch.zeros_like(E0)
+    for k in range(5):
+        o = O.OracleModel(nodes, el, O.Hexa2, "solid", O.iso3d_C(Er, torch.tensor(0.3)))
+        trac = torch.randn(9, 3, generator=torch.Generator().manual_seed(k), dtype=DT)
+        lo, _ = o.integrate_surface_load(top.cpu(), trac)
+        lo = lo / lo.sum(0).norm()
+        loads.append(lo)
+        uo, _, _, _ = o.solve(model.constraints.cpu(), lo, torch.zeros_like(lo), params=(Er,))
+        go += torch.autograd.grad(torch.inner(lo.ravel(), uo.ravel()), Er)[0]
+        oracle_u.append(uo.detach())
+    F = torch.stack(loads).cuda()
+    u, f = model.solve_load_cases(F, differentiable_parameters=Eg, precision=precision, stol=1e-12)
+    assert u.shape == (5, model.n_nod, 3) and f.shape == u.shape
+    for k in range(5):
+        assert rel(u[k], oracle_u[k]) < 1e-9
+    C = (F * u).sum()
+    (g,) = torch.autograd.grad(C, Eg)
+    assert rel(g, go) < 1e-8
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_sixteen_load_cases_match_single_solves(precision):
+    """16 right-hand sides (the batch limit) on a Hexa1 cube: the batched solve
+    agrees with 16 separate FEM.solve() calls, and the force gradient of a
+    weighted sum of displacements with the separate adjoints."""
+    nodes, el = cube_hexa(6, 6, 6)
+    model = Solid(nodes.cuda(), el.cuda(), IsotropicElasticity3D(1000.0, 0.3))
+    X = model.nodes
+    model.constraints[X[:, 0] == 0.0] = True
+    gen = torch.Generator().manual_seed(3)
+    F = torch.randn(16, model.n_nod, 3, generator=gen, dtype=DT).cuda()
+    F.requires_grad_(True)
+    W = torch.randn(16, model.n_nod, 3, generator=gen, dtype=DT).cuda()
+    u, _ = model.solve_load_cases(F, differentiable_parameters=F, precision=precision, stol=1e-12)
+    (gF,) = torch.autograd.grad((W * u).sum(), F)
+    for k in (0, 7, 15):
+        Fk = F[k].detach().clone().requires_grad_(True)
+        model.forces = Fk
+        uk, _, _, _, _ = model.solve(differentiable_parameters=Fk, precision="fp64", stol=1e-12)
+        (gk,) = torch.autograd.grad((W[k] * uk).sum(), Fk)
+        assert rel(u[k], uk) < 1e-9
+        assert rel(gF[k], gk) < 1e-8
+
+
+def test_wide_grid_solves_agree():
+    """A 28^3 Hexa1 cube (24,389 nodes): the FP64 SpMV runs one node per team
+    on more blocks than the partial-sum floor of 2048, and the mixed solver on
+    a capped grid; both reach the tolerance and agree."""
+    nodes, el = cube_hexa(29, 29, 29)
+    model = Solid(nodes.cuda(), el.cuda(), IsotropicElasticity3D(1000.0, 0.3))
+    X = model.nodes
+    model.constraints[X[:, 2] == 0.0, :] = True
+    model.constraints[X[:, 2] == 1.0, 2] = True
+    model.displacements[X[:, 2] == 1.0, 2] = -0.01
+    out = {}
+    for precision in PRECISIONS:
+        u, f, _, _, _ = model.solve(precision=precision, stol=1e-10)
+        info = model.last_solve_info["forward"]["pcg"][-1]
+        assert info["relres"][0] <= 1e-10
+        out[precision] = u
+    assert rel(out["mixed"], out["fp64"]) < 1e-8

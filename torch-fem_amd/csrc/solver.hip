@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <functional>
 #include <type_traits>
 #include <vector>
@@ -29,31 +30,39 @@
 namespace tfm {
 
 constexpr int BS = 256;   // threads per block of the solver kernels
-constexpr int RB = 1024;  // max blocks of a reducing kernel (= partials per reduction)
+constexpr int RBA = 2048; // max blocks of the SpMV kernels (= eta partials per rhs)
+constexpr int RBV = 512;  // max blocks of the vector kernels (= partials per reduction)
 constexpr int MAXR = 16;  // right-hand sides per solve
 constexpr int NSCAL = 4;  // per-iteration scalars: rz, pq, rr, spare
 enum { S_RZ = 0, S_PQ = 1, S_RR = 2 };
 
 // ---------------------------------------------------------- reductions
 // Block partials: thread t contributes v[k] to right-hand side t % NRP; the
-// block's sums go to part[blockIdx.x][k][r].
+// block's sums go to part[blockIdx.x][k][r]. Each wave first folds its lanes
+// of equal t % NRP with a butterfly, then one LDS stage adds the waves (a
+// fixed order, so the result is reproducible).
 template <int NV, int NRP>
 __device__ __forceinline__ void write_block_partials(const double (&v)[NV], double *sh,
                                                      double *part) {
-  const int tid = threadIdx.x;
-  __syncthreads();
+  constexpr int NW = BS / 64;
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  double w[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) sh[k * BS + tid] = v[k];
-  __syncthreads();
-  for (int s = BS / 2; s >= NRP; s >>= 1) {
-    if (tid < s)
+  for (int k = 0; k < NV; ++k) w[k] = v[k];
 #pragma unroll
-      for (int k = 0; k < NV; ++k) sh[k * BS + tid] += sh[k * BS + tid + s];
-    __syncthreads();
-  }
+  for (int off = NRP; off < 64; off <<= 1)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) w[k] += __shfl_xor(w[k], off, 64);
+  __syncthreads();  // sh may still be read by the caller
+  if (lane < NRP)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[(wave * NV + k) * NRP + lane] = w[k];
+  __syncthreads();
   if (tid < NV * NRP) {
-    const int k = tid / NRP, r = tid % NRP;
-    part[(size_t)blockIdx.x * NV * NRP + tid] = sh[k * BS + r];
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) t += sh[q * NV * NRP + tid];
+    part[(size_t)blockIdx.x * NV * NRP + tid] = t;
   }
 }
 
@@ -81,109 +90,132 @@ __device__ __forceinline__ void sum_partials(const double *part, int nblk, doubl
 }
 
 // ------------------------------------------------------- node-block SpMV
+// K contiguous values loaded as one unit (one global_load_dwordxN per lane
+// where the width allows; the compiler otherwise splits rows unevenly).
+template <typename T, int K> struct Vec {
+  T v[K];
+};
+
+template <typename T, int K>
+__device__ __forceinline__ Vec<T, K> ldv(const T *p) {
+  Vec<T, K> r;
+  __builtin_memcpy(&r, p, sizeof(r));
+  return r;
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void stv(T *p, const Vec<T, K> &v) {
+  __builtin_memcpy(p, &v, sizeof(v));
+}
+
+// Neighbour blocks per lane whose loads are issued before any FMA: two for
+// 16-lane teams (a 27-block Hexa1 row in one pass), one for 32-lane teams.
+template <int TL, int NRP> constexpr int hoisted_blocks() { return (TL < 32 && NRP <= 2) ? 2 : 1; }
+
 // A team of TL lanes per node; lane l owns the neighbour blocks p = l, l+TL,
-// ...: it loads adj[p], the ND x NRP entries of x at that neighbour, and the
-// ND x ND block of values (ND contiguous values per row), so each lane does
-// ND*ND*NRP FMAs per one index load. Team sums are butterfly-reduced.
-// Optionally writes per-block partials of dot = sum_i y_i x_i per rhs.
+// ...: it loads adj[p], the ND rows of ND values of the block, and the
+// ND x NRP entries of x at that neighbour (contiguous in the interleaved
+// layout), so each lane does ND*ND*NRP FMAs per index load. The first KB
+// blocks of every lane are loaded before any arithmetic, so a lane keeps
+// KB*(ND+1) loads in flight. The butterfly leaves the team sums in every
+// lane; output element e = a*NRP + r is handled by lane e % TL.
 template <typename T, int ND, int TL, int NRP>
-__global__ void __launch_bounds__(BS) k_spmv(tfm_mesh_t m, const T *__restrict__ val,
-                                             const T *__restrict__ x, T *__restrict__ y,
-                                             const int *alldone, double *part_pq) {
-  if (alldone && *alldone) return;
-  constexpr int TPB = BS / TL;
-  __shared__ double sh[BS > TPB * NRP ? BS : TPB * NRP];
-  const int tid = threadIdx.x, lane = tid % TL, team = tid / TL;
-  double dot[NRP];
+__device__ __forceinline__ void node_product(const tfm_mesh_t &m, const T *__restrict__ val,
+                                             const T *__restrict__ x, int64_t n, int lane,
+                                             T (&acc)[ND][NRP]) {
+  constexpr int KB = hoisted_blocks<TL, NRP>();
 #pragma unroll
-  for (int r = 0; r < NRP; ++r) dot[r] = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * TPB;
-  for (int64_t n = (int64_t)blockIdx.x * TPB + team; n < m.n_nod; n += stride) {
-    const int a0 = m.adj_ptr[n];
-    const int deg = m.adj_ptr[n + 1] - a0;
-    const T *vn = val + m.val_ptr[n];
-    T acc[ND][NRP];
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int r = 0; r < NRP; ++r) acc[a][r] = T(0);
+  const int a0 = m.adj_ptr[n];
+  const int deg = m.adj_ptr[n + 1] - a0;
+  const T *vn = val + m.val_ptr[n];
+  auto fma_block = [&](const Vec<T, ND> (&rows)[ND], const Vec<T, ND * NRP> &xv) {
 #pragma unroll
     for (int a = 0; a < ND; ++a)
 #pragma unroll
-      for (int r = 0; r < NRP; ++r) acc[a][r] = T(0);
-    if (deg > 0) {
-      const int L = ND * deg;
-      for (int p = lane; p < deg; p += TL) {
-        const int64_t col = (int64_t)m.adj[a0 + p] * ND;
-        T xv[ND][NRP];
+      for (int b = 0; b < ND; ++b)
 #pragma unroll
-        for (int b = 0; b < ND; ++b)
+        for (int r = 0; r < NRP; ++r) acc[a][r] += rows[a].v[b] * xv.v[b * NRP + r];
+  };
+  if (deg > 0) {
+    const int L = ND * deg;
+    int col[KB];
+    Vec<T, ND> rows[KB][ND];
 #pragma unroll
-          for (int r = 0; r < NRP; ++r) xv[b][r] = x[(col + b) * NRP + r];
+    for (int k = 0; k < KB; ++k) {
+      const int p = lane + k * TL;
+      col[k] = -1;
+      if (p < deg) {
+        col[k] = m.adj[a0 + p];
 #pragma unroll
-        for (int a = 0; a < ND; ++a)
-#pragma unroll
-          for (int b = 0; b < ND; ++b) {
-            const T v = vn[a * L + p * ND + b];
-#pragma unroll
-            for (int r = 0; r < NRP; ++r) acc[a][r] += v * xv[b][r];
-          }
+        for (int a = 0; a < ND; ++a) rows[k][a] = ldv<T, ND>(vn + a * L + p * ND);
       }
-    } else if (lane < ND) {  // orphan node: only its diagonal is stored
-#pragma unroll
-      for (int a = 0; a < ND; ++a)
-        if (a == lane)
-#pragma unroll
-          for (int r = 0; r < NRP; ++r) acc[a][r] = vn[a] * x[(n * ND + a) * NRP + r];
     }
 #pragma unroll
-    for (int off = 1; off < TL; off <<= 1)
+    for (int k = 0; k < KB; ++k)
+      if (col[k] >= 0) fma_block(rows[k], ldv<T, ND * NRP>(x + (int64_t)col[k] * ND * NRP));
+    for (int p = lane + KB * TL; p < deg; p += TL) {  // rows wider than the hoisted blocks
+      Vec<T, ND> rw[ND];
 #pragma unroll
-      for (int a = 0; a < ND; ++a)
-#pragma unroll
-        for (int r = 0; r < NRP; ++r) acc[a][r] += __shfl_xor(acc[a][r], off, TL);
-    if (lane == 0) {
-#pragma unroll
-      for (int a = 0; a < ND; ++a)
-#pragma unroll
-        for (int r = 0; r < NRP; ++r) {
-          const int64_t o = (n * ND + a) * NRP + r;
-          y[o] = acc[a][r];
-          if (part_pq) dot[r] += (double)acc[a][r] * (double)x[o];
-        }
+      for (int a = 0; a < ND; ++a) rw[a] = ldv<T, ND>(vn + a * L + p * ND);
+      fma_block(rw, ldv<T, ND * NRP>(x + (int64_t)m.adj[a0 + p] * ND * NRP));
     }
+  } else if (lane < ND) {  // orphan node: only its diagonal is stored
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+      if (a == lane)
+#pragma unroll
+        for (int r = 0; r < NRP; ++r) acc[a][r] = vn[a] * x[(n * ND + a) * NRP + r];
   }
-  if (!part_pq) return;
-  // team leaders hold the dots; regroup so thread t carries rhs t % NRP
-  __syncthreads();
-  if (lane == 0)
 #pragma unroll
-    for (int r = 0; r < NRP; ++r) sh[team * NRP + r] = dot[r];
-  __syncthreads();
-  double v[1] = {tid < TPB * NRP ? sh[tid] : 0.0};
-  write_block_partials<1, NRP>(v, sh, part_pq);
+  for (int off = 1; off < TL; off <<= 1)
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int r = 0; r < NRP; ++r) acc[a][r] += __shfl_xor(acc[a][r], off, TL);
+}
+
+// y = K x over the node blocks (every right-hand side of the interleaved x).
+template <typename T, int ND, int TL, int NRP>
+__global__ void __launch_bounds__(BS) k_spmv(tfm_mesh_t m, const T *__restrict__ val,
+                                             const T *__restrict__ x, T *__restrict__ y) {
+  constexpr int TPB = BS / TL;
+  const int lane = threadIdx.x % TL, team = threadIdx.x / TL;
+  for (int64_t n = (int64_t)blockIdx.x * TPB + team; n < m.n_nod; n += (int64_t)gridDim.x * TPB) {
+    T acc[ND][NRP];
+    node_product<T, ND, TL, NRP>(m, val, x, n, lane, acc);
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int r = 0; r < NRP; ++r)
+        if ((a * NRP + r) % TL == lane) y[(n * ND + a) * NRP + r] = acc[a][r];
+  }
 }
 
 // ------------------------------------------------------- general CSR SpMV
 template <int TL>
+__device__ __forceinline__ double csr_row(const int64_t *rowptr, const int32_t *col,
+                                          const double *val, const double *x, int64_t i,
+                                          int lane) {
+  double acc = 0.0;
+  for (int64_t k = rowptr[i] + lane; k < rowptr[i + 1]; k += TL) acc += val[k] * x[col[k]];
+#pragma unroll
+  for (int off = 1; off < TL; off <<= 1) acc += __shfl_xor(acc, off, TL);
+  return acc;
+}
+
+template <int TL>
 __global__ void __launch_bounds__(BS) k_spmv_csr(int64_t n, const int64_t *rowptr,
                                                  const int32_t *col, const double *val,
-                                                 const double *x, double *y, const int *alldone,
-                                                 double *part_pq) {
-  if (alldone && *alldone) return;
+                                                 const double *x, double *y) {
   constexpr int TPB = BS / TL;
-  __shared__ double sh[BS];
-  const int tid = threadIdx.x, lane = tid % TL, team = tid / TL;
-  double dot = 0.0;
+  const int lane = threadIdx.x % TL, team = threadIdx.x / TL;
   for (int64_t i = (int64_t)blockIdx.x * TPB + team; i < n; i += (int64_t)gridDim.x * TPB) {
-    double acc = 0.0;
-    for (int64_t k = rowptr[i] + lane; k < rowptr[i + 1]; k += TL) acc += val[k] * x[col[k]];
-#pragma unroll
-    for (int off = 1; off < TL; off <<= 1) acc += __shfl_xor(acc, off, TL);
-    if (lane == 0) {
-      y[i] = acc;
-      dot += acc * x[i];
-    }
+    const double acc = csr_row<TL>(rowptr, col, val, x, i, lane);
+    if (lane == 0) y[i] = acc;
   }
-  if (!part_pq) return;
-  double v[1] = {dot};
-  write_block_partials<1, 1>(v, sh, part_pq);
 }
 
 // ------------------------------------------------------- preconditioner
@@ -243,79 +275,222 @@ __device__ __forceinline__ void apply_pc(const T *minv, const T *r, int64_t n, i
 }
 
 // ------------------------------------------------------------ PCG steps
-struct PcgState {
-  double *scal;      // [(max_iter + 2) * NRP * NSCAL]
-  double *tol2;      // [NRP] stop the current segment when rr <= tol2 (< 0: never live)
-  int *done;         // [NRP] iteration at which rhs r stopped, -1 while running
-  int *alldone;      // 1 once every rhs stopped
-  double *part_pq;   // [RB * NRP]        written by the SpMV
-  double *part_upd;  // [RB * 2 * NRP]    written by the update
-  double *part_aux;  // [RB * 3 * NRP]    start / reliable-update reductions
+// One iteration is two kernels, the form of PCG that applies the operator to
+// the preconditioned residual u = M r instead of to the search direction:
+//   A (spmv_dir): gamma = r.u and rr = r.r from the partials of B; stop test;
+//                 beta = gamma / gamma_prev; w = K u; p = u + beta p;
+//                 s = w + beta s (so that s = K p); partials of eta = p.s
+//   B (update):   alpha = gamma / eta; x += alpha p; r -= alpha s; u = M r;
+//                 partials of (r.u, r.r)
+// In exact arithmetic these are the iterates of textbook PCG; eta is a dot
+// product of the stored vectors (not a recurrence), and every dot is FP64.
+// Per-rhs coefficients prepared by a one-block scalar kernel for a wide-grid
+// kernel (beta for A, alpha for B), read once per block instead of every
+// block summing partial sets; `any` is 0 once no rhs is running.
+struct PcgCoef {
+  double c[MAXR];
+  int live[MAXR];
+  int any;
+  int pad;
 };
 
-// z = M r; p = z; partials of (r.z, r.r) for a (re)start from the residual in r.
-template <typename T, int ND, int NRP>
-__global__ void __launch_bounds__(BS) k_pcg_start(int64_t n_nod, const T *r, T *z, T *p,
-                                                  const T *minv, PcgState st) {
-  __shared__ double sh[2 * BS];
+struct PcgState {
+  double *scal;      // [(max_iter + 2) * NRP * NSCAL]: gamma, eta, rr per iteration
+  double *tol2;      // [NRP] stop the current segment when rr <= tol2 (< 0: rhs not running)
+  double *rzold;     // [NRP] gamma that divides the first beta of a segment (0: restart)
+  int *done;         // [NRP] iteration at which rhs r stopped, -1 while running
+  int *alldone;      // 1 once every rhs stopped
+  double *part_pq;   // [pq_cap * NRP]   eta partials (A)
+  double *part_upd;  // [RBV * 2 * NRP]  (gamma, rr) partials (B and the segment start)
+  double *part_aux;  // [RBV * NRP]      r.p partials (segment start)
+  int pq_cap;        // blocks of A whose partials part_pq can hold
+  PcgCoef *coef_a;   // beta of A (scalar-kernel mode)
+  PcgCoef *coef_b;   // alpha of B (scalar-kernel mode)
+};
+
+// Host view of where every rhs stopped, passed by value to the segment start.
+struct SegInfo {
+  int d[MAXR];
+};
+
+// Prologue of A: the scalars of iteration `it`, identical in every block (the
+// partials are summed in block order); block 0 records them and the stop
+// flags. Returns nonzero while some rhs is running.
+template <int NRP>
+__device__ __forceinline__ int dir_prologue(int it, int seg_first, const double *part_in, int n_in,
+                                            const PcgState &st, double *sh, double *tot,
+                                            double *beta, int *live) {
+  sum_partials<2 * NRP>(part_in, n_in, sh, tot);
   const int tid = threadIdx.x;
-  double acc[2] = {0.0, 0.0};
-  const int64_t total = n_nod * NRP;
-  for (int64_t t = (int64_t)blockIdx.x * BS + tid; t < total; t += (int64_t)gridDim.x * BS) {
-    const int64_t n = t / NRP;
-    const int rr = (int)(t % NRP);
-    T rv[ND], zv[ND];
-    apply_pc<T, ND, NRP>(minv, r, n, rr, zv, rv);
-#pragma unroll
-    for (int a = 0; a < ND; ++a) {
-      z[(n * ND + a) * NRP + rr] = zv[a];
-      p[(n * ND + a) * NRP + rr] = zv[a];
-      acc[0] += (double)rv[a] * (double)zv[a];
-      acc[1] += (double)rv[a] * (double)rv[a];
+  if (tid < NRP) {
+    const double g = tot[tid], rr = tot[NRP + tid];
+    // a concurrent write of done[] by block 0 only turns -1 into `it`, and
+    // only for a rhs that stops now: every block sees it as not running
+    const bool was = st.done[tid] >= 0;
+    const bool now = !was && (rr <= st.tol2[tid] || !(rr == rr));
+    const bool lv = !was && !now;
+    const double den = it == seg_first ? st.rzold[tid]
+                                       : st.scal[((size_t)(it - 1) * NRP + tid) * NSCAL + S_RZ];
+    live[tid] = lv;
+    beta[tid] = (lv && den != 0.0) ? g / den : 0.0;
+    if (blockIdx.x == 0) {
+      double *sc = st.scal + ((size_t)it * NRP + tid) * NSCAL;
+      sc[S_RZ] = g;
+      sc[S_RR] = rr;
+      if (now) st.done[tid] = it;
     }
   }
-  write_block_partials<2, NRP>(acc, sh, st.part_aux);
-}
-
-// One block: rz and rr of the (re)start at iteration `it`; live flags.
-template <int NRP>
-__global__ void __launch_bounds__(BS) k_pcg_start_finish(int it, int nblk, PcgState st) {
-  __shared__ double sh[BS];
-  __shared__ double tot[2 * NRP];
-  sum_partials<2 * NRP>(st.part_aux, nblk, sh, tot);
-  const int tid = threadIdx.x;
-  __shared__ int live[NRP];
-  if (tid < NRP) {
-    double *sc = st.scal + ((size_t)it * NRP + tid) * NSCAL;
-    sc[S_RZ] = tot[tid];
-    sc[S_RR] = tot[NRP + tid];
-    const bool lv = st.tol2[tid] >= 0.0 && tot[NRP + tid] > st.tol2[tid];
-    st.done[tid] = lv ? -1 : it;
-    live[tid] = lv;
-  }
   __syncthreads();
-  if (tid == 0) {
-    int any = 0;
-    for (int r = 0; r < NRP; ++r) any |= live[r];
-    *st.alldone = !any;
-  }
+  int any = 0;
+#pragma unroll
+  for (int r = 0; r < NRP; ++r) any |= live[r];
+  if (blockIdx.x == 0 && tid == 0) *st.alldone = !any;
+  return any;
 }
 
-// alpha = rz/pq; x += alpha p; r -= alpha q; z = M r; partials of (r.z, r.r).
-template <typename T, int ND, int NRP>
-__global__ void __launch_bounds__(BS) k_pcg_update(int64_t n_nod, int it, T *x, T *r, T *z,
-                                                   const T *p, const T *q, const T *minv,
-                                                   PcgState st, int n_pq) {
+// A over the node blocks. TL is a multiple of NRP, so the lane that updates
+// element e = a*NRP + r (lane e % TL) belongs to rhs tid % NRP, the slot of
+// its eta partial. CO: beta and the running flags come from coef_a (prepared
+// by k_pcg_scal_dir) instead of the partial sets at part_in.
+template <typename T, int ND, int TL, int NRP, bool CO>
+__device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__restrict__ val,
+                                              const T *__restrict__ u, T *__restrict__ p,
+                                              T *__restrict__ s, int it, int seg_first,
+                                              const double *part_in, int n_in,
+                                              const PcgState &st) {
+  static_assert(TL % NRP == 0, "team width must be a multiple of the rhs count");
+  __shared__ double sh[BS];
+  __shared__ double tot[2 * NRP], beta[NRP];
+  __shared__ int live[NRP];
+  if constexpr (CO) {
+    const PcgCoef &cf = *st.coef_a;
+    if (!cf.any) return;
+    if (threadIdx.x < NRP) {
+      beta[threadIdx.x] = cf.c[threadIdx.x];
+      live[threadIdx.x] = cf.live[threadIdx.x];
+    }
+    __syncthreads();
+  } else {
+    if (*st.alldone) return;
+    if (!dir_prologue<NRP>(it, seg_first, part_in, n_in, st, sh, tot, beta, live)) return;
+  }
+  constexpr int TPB = BS / TL;
+  // lane l updates the elements e = l, l + TL, ... of the node's ND*NRP
+  // (offset e from the node's first entry); they all belong to rhs l % NRP
+  constexpr int NE = ND * NRP, KE = (NE + TL - 1) / TL;
+  const int lane = threadIdx.x % TL, team = threadIdx.x / TL;
+  const T b = (T)beta[lane % NRP];
+  const bool lv = live[lane % NRP] != 0;
+  double dot = 0.0;
+  for (int64_t n = (int64_t)blockIdx.x * TPB + team; n < m.n_nod; n += (int64_t)gridDim.x * TPB) {
+    const int64_t nb = n * NE;
+    T up[KE], pp[KE], sp[KE];  // issued with the SpMV loads, used after it
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int e = lane + k * TL;
+      if (lv && e < NE) {
+        up[k] = u[nb + e];
+        pp[k] = p[nb + e];
+        sp[k] = s[nb + e];
+      }
+    }
+    T acc[ND][NRP];
+    node_product<T, ND, TL, NRP>(m, val, u, n, lane, acc);
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int e = lane + k * TL;
+      if (lv && e < NE) {
+        T w = T(0);
+#pragma unroll
+        for (int a = 0; a < ND; ++a)
+#pragma unroll
+          for (int r = 0; r < NRP; ++r)
+            if (a * NRP + r == e) w = acc[a][r];
+        const T pv = up[k] + b * pp[k];
+        const T sv = w + b * sp[k];
+        p[nb + e] = pv;
+        s[nb + e] = sv;
+        dot += (double)pv * (double)sv;
+      }
+    }
+  }
+  double v[1] = {dot};
+  write_block_partials<1, NRP>(v, sh, st.part_pq);
+}
+
+template <typename T, int ND, int TL, int NRP, bool CO>
+__global__ void __launch_bounds__(BS) k_pcg_spmv_dir(tfm_mesh_t m, const T *__restrict__ val,
+                                                     const T *__restrict__ u, T *__restrict__ p,
+                                                     T *__restrict__ s, int it, int seg_first,
+                                                     const double *part_in, int n_in,
+                                                     PcgState st) {
+  spmv_dir_body<T, ND, TL, NRP, CO>(m, val, u, p, s, it, seg_first, part_in, n_in, st);
+}
+
+// The same kernel held to 64 VGPRs (8 waves per SIMD), so that a grid of
+// 2048 blocks of 4 waves is resident in one round on 256 CUs.
+template <typename T, int ND, int TL, int NRP, bool CO>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_pcg_spmv_dir_w8(tfm_mesh_t m, const T *__restrict__ val, const T *__restrict__ u,
+                  T *__restrict__ p, T *__restrict__ s, int it, int seg_first,
+                  const double *part_in, int n_in, PcgState st) {
+  spmv_dir_body<T, ND, TL, NRP, CO>(m, val, u, p, s, it, seg_first, part_in, n_in, st);
+}
+
+// A over a general CSR matrix (one rhs).
+template <int TL>
+__global__ void __launch_bounds__(BS) k_pcg_csr_dir(int64_t n, const int64_t *rowptr,
+                                                    const int32_t *col, const double *val,
+                                                    const double *u, double *p, double *s, int it,
+                                                    int seg_first, const double *part_in, int n_in,
+                                                    PcgState st) {
   if (*st.alldone) return;
+  __shared__ double sh[BS];
+  __shared__ double tot[2], beta[1];
+  __shared__ int live[1];
+  if (!dir_prologue<1>(it, seg_first, part_in, n_in, st, sh, tot, beta, live)) return;
+  constexpr int TPB = BS / TL;
+  const int lane = threadIdx.x % TL, team = threadIdx.x / TL;
+  double dot = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * TPB + team; i < n; i += (int64_t)gridDim.x * TPB) {
+    const double w = csr_row<TL>(rowptr, col, val, u, i, lane);
+    if (lane == 0) {
+      const double pv = u[i] + beta[0] * p[i];
+      const double sv = w + beta[0] * s[i];
+      p[i] = pv;
+      s[i] = sv;
+      dot += pv * sv;
+    }
+  }
+  double v[1] = {dot};
+  write_block_partials<1, 1>(v, sh, st.part_pq);
+}
+
+// B: alpha = gamma / eta; x += alpha p; r -= alpha s; u = M r; partials of
+// (r.u, r.r) for the next A. pq_in holds n_pq partial sets of eta.
+template <typename T, int ND, int NRP, bool CO>
+__global__ void __launch_bounds__(BS) k_pcg_update(int64_t n_nod, int it, T *__restrict__ x,
+                                                   T *__restrict__ r, T *__restrict__ u,
+                                                   const T *__restrict__ p,
+                                                   const T *__restrict__ s,
+                                                   const T *__restrict__ minv, PcgState st,
+                                                   const double *pq_in, int n_pq) {
   __shared__ double sh[2 * BS];
   __shared__ double pq[NRP];
   __shared__ double alpha[NRP];
-  sum_partials<NRP>(st.part_pq, n_pq, sh, pq);
   const int tid = threadIdx.x;
-  if (tid < NRP) {
-    double *sc = st.scal + ((size_t)it * NRP + tid) * NSCAL;
-    alpha[tid] = (st.done[tid] < 0 && pq[tid] != 0.0) ? sc[S_RZ] / pq[tid] : 0.0;
-    if (blockIdx.x == 0) sc[S_PQ] = pq[tid];
+  if constexpr (CO) {  // alpha prepared by k_pcg_scal_alpha
+    const PcgCoef &cf = *st.coef_b;
+    if (!cf.any) return;
+    if (tid < NRP) alpha[tid] = cf.c[tid];
+  } else {
+    if (*st.alldone) return;
+    sum_partials<NRP>(pq_in, n_pq, sh, pq);
+    if (tid < NRP) {
+      double *sc = st.scal + ((size_t)it * NRP + tid) * NSCAL;
+      alpha[tid] = (st.done[tid] < 0 && pq[tid] != 0.0) ? sc[S_RZ] / pq[tid] : 0.0;
+      if (blockIdx.x == 0) sc[S_PQ] = pq[tid];
+    }
   }
   __syncthreads();
   double acc[2] = {0.0, 0.0};
@@ -324,71 +499,113 @@ __global__ void __launch_bounds__(BS) k_pcg_update(int64_t n_nod, int it, T *x, 
     const int64_t n = t / NRP;
     const int rr = (int)(t % NRP);
     const T al = (T)alpha[rr];
-#pragma unroll
-    for (int a = 0; a < ND; ++a) {
-      const int64_t o = (n * ND + a) * NRP + rr;
-      x[o] += al * p[o];
-      r[o] -= al * q[o];
-    }
+    const Vec<T, ND * ND> mi = ldv<T, ND * ND>(minv + n * ND * ND);
     T rv[ND], zv[ND];
-    apply_pc<T, ND, NRP>(minv, r, n, rr, zv, rv);
+    if constexpr (NRP == 1) {  // the node's ND entries are contiguous
+      Vec<T, ND> xv = ldv<T, ND>(x + n * ND), rw = ldv<T, ND>(r + n * ND);
+      const Vec<T, ND> pv = ldv<T, ND>(p + n * ND), sv = ldv<T, ND>(s + n * ND);
+#pragma unroll
+      for (int a = 0; a < ND; ++a) {
+        xv.v[a] += al * pv.v[a];
+        rw.v[a] -= al * sv.v[a];
+        rv[a] = rw.v[a];
+      }
+      stv<T, ND>(x + n * ND, xv);
+      stv<T, ND>(r + n * ND, rw);
+    } else {
+#pragma unroll
+      for (int a = 0; a < ND; ++a) {
+        const int64_t o = (n * ND + a) * NRP + rr;
+        x[o] += al * p[o];
+        rv[a] = r[o] - al * s[o];
+        r[o] = rv[a];
+      }
+    }
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
-      z[(n * ND + a) * NRP + rr] = zv[a];
-      acc[0] += (double)rv[a] * (double)zv[a];
+      T z = T(0);
+#pragma unroll
+      for (int b2 = 0; b2 < ND; ++b2) z += mi.v[a * ND + b2] * rv[b2];
+      zv[a] = z;
+      acc[0] += (double)rv[a] * (double)z;
       acc[1] += (double)rv[a] * (double)rv[a];
+    }
+    if constexpr (NRP == 1) {
+      Vec<T, ND> zo;
+#pragma unroll
+      for (int a = 0; a < ND; ++a) zo.v[a] = zv[a];
+      stv<T, ND>(u + n * ND, zo);
+    } else {
+#pragma unroll
+      for (int a = 0; a < ND; ++a) u[(n * ND + a) * NRP + rr] = zv[a];
     }
   }
   write_block_partials<2, NRP>(acc, sh, st.part_upd);
 }
 
-// beta = rz_{it+1}/rz_it; stop test on rr_{it+1}; p = z + beta p. Block 0
-// records the scalars and the stop flags for the host and the next kernels.
-template <typename T, int ND, int NRP>
-__global__ void __launch_bounds__(BS) k_pcg_dir(int64_t n_nod, int it, const T *z, T *p,
-                                                PcgState st, int n_upd) {
-  if (*st.alldone) return;
+// Scalar kernels (one block) for wide grids. k_pcg_scal_dir prepares A of
+// iteration `it`: the same scalars, stop test and records as the prologue of
+// A (dir_prologue, run here by a single block), published in coef_a.
+template <int NRP>
+__global__ void __launch_bounds__(BS) k_pcg_scal_dir(int it, int seg_first, const double *part_in,
+                                                     int n_in, PcgState st) {
   __shared__ double sh[BS];
-  __shared__ double tot[2 * NRP];
-  __shared__ double beta[NRP];
+  __shared__ double tot[2 * NRP], beta[NRP];
   __shared__ int live[NRP];
-  sum_partials<2 * NRP>(st.part_upd, n_upd, sh, tot);
+  const int any = dir_prologue<NRP>(it, seg_first, part_in, n_in, st, sh, tot, beta, live);
   const int tid = threadIdx.x;
   if (tid < NRP) {
-    const double rz1 = tot[tid], rr1 = tot[NRP + tid];
-    const double rz0 = st.scal[((size_t)it * NRP + tid) * NSCAL + S_RZ];
-    const bool was = st.done[tid] >= 0;
-    const bool now = !was && (rr1 <= st.tol2[tid] || !(rr1 == rr1));
-    live[tid] = !was && !now;
-    beta[tid] = rz0 != 0.0 ? rz1 / rz0 : 0.0;
-    if (blockIdx.x == 0) {
-      double *sc = st.scal + ((size_t)(it + 1) * NRP + tid) * NSCAL;
-      sc[S_RZ] = rz1;
-      sc[S_RR] = rr1;
-      if (now) st.done[tid] = it + 1;
-    }
+    st.coef_a->c[tid] = beta[tid];
+    st.coef_a->live[tid] = live[tid];
   }
-  __syncthreads();
-  if (blockIdx.x == 0 && tid == 0) {
-    int any = 0;
-    for (int r = 0; r < NRP; ++r) any |= live[r];
-    *st.alldone = !any;
-  }
-  const int64_t total = n_nod * ND * NRP;
-  for (int64_t t = (int64_t)blockIdx.x * BS + tid; t < total; t += (int64_t)gridDim.x * BS) {
-    const int rr = (int)(t % NRP);
-    if (live[rr]) p[t] = z[t] + (T)beta[rr] * p[t];
-  }
+  if (tid == 0) st.coef_a->any = any;
 }
 
-// Reliable update, part 1: z = M r for the replaced residuals; partials of
-// (r.z, r.p, r.r).
-template <typename T, int ND, int NRP>
-__global__ void __launch_bounds__(BS) k_reliable_dots(int64_t n_nod, const T *r, T *z, const T *p,
-                                                      const T *minv, PcgState st) {
-  __shared__ double sh[3 * BS];
+// k_pcg_scal_alpha prepares B of iteration `it`: eta from the nblk partial
+// sets of A (summed in block order by 1024 threads, four loads in flight per
+// thread), alpha = gamma / eta, published in coef_b.
+template <int NRP>
+__global__ void __launch_bounds__(1024) k_pcg_scal_alpha(int it, const double *part, int nblk,
+                                                         PcgState st) {
+  __shared__ double sh[1024];
+  constexpr int CH = 1024 / NRP;
   const int tid = threadIdx.x;
-  double acc[3] = {0.0, 0.0, 0.0};
+  const int slot = tid % NRP, c = tid / NRP;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int b = c;
+  for (; b + 3 * CH < nblk; b += 4 * CH) {
+    a0 += part[(size_t)b * NRP + slot];
+    a1 += part[(size_t)(b + CH) * NRP + slot];
+    a2 += part[(size_t)(b + 2 * CH) * NRP + slot];
+    a3 += part[(size_t)(b + 3 * CH) * NRP + slot];
+  }
+  for (; b < nblk; b += CH) a0 += part[(size_t)b * NRP + slot];
+  sh[tid] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  for (int w = CH / 2; w >= 1; w >>= 1) {
+    if (tid < w * NRP) sh[tid] += sh[tid + w * NRP];
+    __syncthreads();
+  }
+  if (tid < NRP) {
+    double *sc = st.scal + ((size_t)it * NRP + tid) * NSCAL;
+    const double pq = sh[tid];
+    const bool lv = st.done[tid] < 0;
+    sc[S_PQ] = pq;
+    st.coef_b->c[tid] = (lv && pq != 0.0) ? sc[S_RZ] / pq : 0.0;
+    st.coef_b->live[tid] = lv;
+  }
+  if (tid == 0) st.coef_b->any = !*st.alldone;
+}
+
+// Segment start, part 1: u = M r for the (replaced) residuals; partials of
+// (r.u, r.r) for the next A and of r.p for the continuation.
+template <typename T, int ND, int NRP>
+__global__ void __launch_bounds__(BS) k_pcg_seg_dots(int64_t n_nod, const T *r, T *u, const T *p,
+                                                     const T *minv, PcgState st) {
+  __shared__ double sh[2 * BS];
+  const int tid = threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  double rp = 0.0;
   const int64_t total = n_nod * NRP;
   for (int64_t t = (int64_t)blockIdx.x * BS + tid; t < total; t += (int64_t)gridDim.x * BS) {
     const int64_t n = t / NRP;
@@ -398,59 +615,46 @@ __global__ void __launch_bounds__(BS) k_reliable_dots(int64_t n_nod, const T *r,
 #pragma unroll
     for (int a = 0; a < ND; ++a) {
       const int64_t o = (n * ND + a) * NRP + rr;
-      z[o] = zv[a];
+      u[o] = zv[a];
       acc[0] += (double)rv[a] * (double)zv[a];
-      acc[1] += (double)rv[a] * (double)p[o];
-      acc[2] += (double)rv[a] * (double)rv[a];
+      acc[1] += (double)rv[a] * (double)rv[a];
+      rp += (double)rv[a] * (double)p[o];
     }
   }
-  write_block_partials<3, NRP>(acc, sh, st.part_aux);
+  write_block_partials<2, NRP>(acc, sh, st.part_upd);
+  double v[1] = {rp};
+  write_block_partials<1, NRP>(v, sh, st.part_aux);
 }
 
-// Reliable update, part 2 (one block): rz of the iteration at which each
-// continuing rhs stopped, the denominator of its next beta (0: restart).
-// A rhs in `restart` restarts from the new residual (beta = 0, p = z).
-template <int NRP>
-__global__ void k_reliable_prep(int mask, int restart, int seg_start, PcgState st, double *rzold) {
-  const int r = threadIdx.x;
-  if (r >= NRP) return;
-  const int d = st.done[r];
-  double v = 0.0;
-  if (((mask >> r) & 1) && !((restart >> r) & 1) && d > seg_start)
-    v = st.scal[((size_t)(d - 1) * NRP + r) * NSCAL + S_RZ];
-  rzold[r] = v;
-}
-
-// Reliable update, part 3: for the continuing rhs (bit set in `mask`) the kept
-// direction is made orthogonal to the new residual, p' = p - (r.p / r.z) z,
-// and the interrupted direction update is completed, p = z + beta p' with
-// beta = (r.z) / rz_old; the scalar recurrence restarts at iteration `it`.
+// Segment start, part 2. A rhs whose bit is set in `mask` runs the next
+// segment. If its bit is also set in `cont`, it keeps its search direction:
+// p' = p - (r.p / r.u) u is orthogonal to the new residual, and the first
+// beta of the segment divides by gamma of the iteration at which it stopped
+// (seg.d - 1); the caller then forms s = K p'. Otherwise it restarts from
+// p = s = 0 (beta = 0). A rhs outside `mask` has finished and is left as is.
 template <typename T, int ND, int NRP>
-__global__ void __launch_bounds__(BS) k_reliable_apply(int64_t n_nod, int it, int mask,
-                                                       const T *z, T *p, PcgState st,
-                                                       const double *rzold, int nblk) {
+__global__ void __launch_bounds__(BS) k_pcg_seg_apply(int64_t n_nod, int it, int mask, int cont,
+                                                      SegInfo seg, const T *u, T *p, T *s,
+                                                      PcgState st, int nblk) {
   __shared__ double sh[BS];
-  __shared__ double tot[3 * NRP];
-  __shared__ double cz[NRP], cp[NRP];
-  __shared__ int live[NRP];
-  sum_partials<3 * NRP>(st.part_aux, nblk, sh, tot);
+  __shared__ double tot[2 * NRP], rp[NRP], cz[NRP];
+  __shared__ int mode[NRP];
+  sum_partials<2 * NRP>(st.part_upd, nblk, sh, tot);
+  sum_partials<NRP>(st.part_aux, nblk, sh, rp);
   const int tid = threadIdx.x;
   if (tid < NRP) {
-    const bool lv = (mask >> tid) & 1;
-    const double rz = tot[tid], rp = tot[NRP + tid];
-    double beta = (lv && rzold[tid] != 0.0) ? rz / rzold[tid] : 0.0;
-    double c = (lv && rz != 0.0) ? rp / rz : 0.0;
-    if (!(beta == beta) || !(c == c)) beta = c = 0.0;
-    live[tid] = lv;
-    cz[tid] = 1.0 - beta * c;  // p = cz z + cp p
-    cp[tid] = beta;
+    const bool run = (mask >> tid) & 1;
+    const int d = seg.d[tid];
+    bool keep = run && ((cont >> tid) & 1) && d >= 1;
+    const double g = tot[tid];
+    const double c = keep && g != 0.0 ? rp[tid] / g : 0.0;
+    const double den = keep ? st.scal[((size_t)(d - 1) * NRP + tid) * NSCAL + S_RZ] : 0.0;
+    if (!(c == c) || !(den == den) || den == 0.0 || g == 0.0) keep = false;
+    mode[tid] = keep ? 1 : (run ? 2 : 0);
+    cz[tid] = keep ? c : 0.0;
     if (blockIdx.x == 0) {
-      double *sc = st.scal + ((size_t)it * NRP + tid) * NSCAL;
-      if (lv) {
-        sc[S_RZ] = rz;
-        sc[S_RR] = tot[2 * NRP + tid];
-      }
-      st.done[tid] = lv ? -1 : (st.done[tid] >= 0 ? st.done[tid] : it);
+      st.rzold[tid] = keep ? den : 0.0;
+      st.done[tid] = run ? -1 : (d >= 0 ? d : it);
     }
   }
   __syncthreads();
@@ -458,7 +662,13 @@ __global__ void __launch_bounds__(BS) k_reliable_apply(int64_t n_nod, int it, in
   const int64_t total = n_nod * ND * NRP;
   for (int64_t t = (int64_t)blockIdx.x * BS + tid; t < total; t += (int64_t)gridDim.x * BS) {
     const int rr = (int)(t % NRP);
-    if (live[rr]) p[t] = (T)cz[rr] * z[t] + (T)cp[rr] * p[t];
+    const int md = mode[rr];
+    if (md == 1) {
+      p[t] -= (T)cz[rr] * u[t];
+    } else if (md == 2) {
+      p[t] = T(0);
+      s[t] = T(0);
+    }
   }
 }
 
@@ -516,32 +726,69 @@ __global__ void __launch_bounds__(BS) k_sum2(const double *part, int nblk, doubl
 }
 
 // ------------------------------------------------------- launch helpers
-static unsigned grid_for(int64_t work, int per_block) {
-  const int64_t g = (work + per_block - 1) / per_block;
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, RB));
+static int env_int(const char *name, int dflt) {
+  const char *v = std::getenv(name);
+  return (v && *v) ? std::atoi(v) : dflt;
 }
 
-// Team width for the node-block SpMV from the mean node degree.
-static int team_width(const tfm_mesh_t &m) {
+static unsigned grid_for(int64_t work, int per_block, int cap) {
+  const int64_t g = (work + per_block - 1) / per_block;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+// Whether to launch the 64-VGPR variant of a kernel: when it needs no scratch
+// (TFM_SPMV_W8=0/1 in the environment forces the choice).
+static bool pick_w8(const void *kernel_w8) {
+  static const int forced = env_int("TFM_SPMV_W8", -1);
+  if (forced >= 0) return forced != 0;
+  hipFuncAttributes attr{};
+  if (hipFuncGetAttributes(&attr, kernel_w8) != hipSuccess) return false;
+  return attr.localSizeBytes == 0;
+}
+
+// Blocks of BS threads of `kernel` resident at once on the current device
+// (occupancy per CU times CUs): the grid of a grid-stride kernel that should
+// run in a single round.
+static int resident_grid(const void *kernel) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, BS, 0) != hipSuccess)
+    return RBA;
+  return std::max(1, per_cu) * std::max(1, cus);
+}
+
+// Launch shape of the node-block SpMV kernels for value type T: the team
+// width, the grid cap, and whether one-block scalar kernels prepare beta and
+// alpha (then A and B read one coefficient record per block instead of
+// summing partial sets, and A's grid is not limited by its consumers).
+// Measured on MI355X with the Hexa1 64^3 matrix: FP32 streams fastest with
+// 16-lane teams (two hoisted blocks per lane) on 2048 blocks, whose partials
+// every block of B can sum directly; FP64 needs the full grid (one node per
+// team, 5.8 TB/s vs 4.7 TB/s at 2048 blocks) and the scalar kernels.
+// Narrower rows (2-D, heat) use 16-lane teams. TFM_SPMV_GRID / TFM_SPMV_SCAL
+// in the environment override the grid cap and the mode (tuning aid).
+struct SpmvShape {
+  int tl;
+  int cap;
+  bool scal;  // one-block scalar kernels prepare the coefficients of A and B
+};
+
+template <typename T>
+static SpmvShape spmv_shape(const tfm_mesh_t &m, int nrp) {
   const int64_t n_adj = m.n_nod > 0 ? (int64_t)m.nnz / std::max(1, m.ndof * m.ndof) : 0;
   const double deg = (double)n_adj / std::max<int64_t>(1, m.n_nod);
-  return deg > 16.0 ? 32 : deg > 8.0 ? 16 : 8;
-}
-
-// Launches the node-block SpMV; returns the grid size (number of partials).
-template <typename T, int ND, int NRP>
-static int spmv_launch(hipStream_t s, const tfm_mesh_t &m, const T *val, const T *x, T *y,
-                       const int *alldone, double *part_pq, unsigned *grid_out) {
-  const int tl = team_width(m);
-  const unsigned g = grid_for(m.n_nod, BS / tl);
-  switch (tl) {
-    case 8: hipLaunchKernelGGL((k_spmv<T, ND, 8, NRP>), dim3(g), dim3(BS), 0, s, m, val, x, y, alldone, part_pq); break;
-    case 16: hipLaunchKernelGGL((k_spmv<T, ND, 16, NRP>), dim3(g), dim3(BS), 0, s, m, val, x, y, alldone, part_pq); break;
-    default: hipLaunchKernelGGL((k_spmv<T, ND, 32, NRP>), dim3(g), dim3(BS), 0, s, m, val, x, y, alldone, part_pq); break;
-  }
-  TFM_CHECK_LAUNCH();
-  if (grid_out) *grid_out = g;
-  return TFM_OK;
+  SpmvShape sh;
+  const bool fp64 = sizeof(T) == 8;
+  sh.tl = (fp64 && deg > 16.0) || deg > 32.0 ? 32 : 16;
+  sh.scal = fp64;
+  sh.cap = fp64 ? (1 << 30) : (nrp <= 2 ? RBA : nrp <= 8 ? RBA / 2 : RBA / 4);
+  static const int g_env = env_int("TFM_SPMV_GRID", 0);
+  static const int f_env = env_int("TFM_SPMV_SCAL", -1);
+  if (g_env > 0) sh.cap = g_env;
+  if (f_env >= 0) sh.scal = f_env != 0;
+  if (!sh.scal) sh.cap = std::min(sh.cap, RBA);  // every block of B sums A's partial sets
+  return sh;
 }
 
 #define TFM_NRP_SWITCH(nrp, ...)                                                               \
@@ -562,17 +809,63 @@ static int spmv_launch(hipStream_t s, const tfm_mesh_t &m, const T *val, const T
     default: TFM_REQUIRE(false, "ndof must be 1, 2 or 3");                                     \
   }
 
+// Team width TL (16 or 32; a multiple of NRP <= 16).
+#define TFM_TEAM_SWITCH(tl, ...)                                                               \
+  switch (tl) {                                                                                \
+    case 16: { constexpr int TL = 16; __VA_ARGS__; break; }                                    \
+    default: { constexpr int TL = 32; __VA_ARGS__; break; }                                    \
+  }
+
+template <typename T, int ND, int NRP>
+static int spmv_launch(hipStream_t s, const tfm_mesh_t &m, const T *val, const T *x, T *y) {
+  const SpmvShape sh = spmv_shape<T>(m, NRP);
+  TFM_TEAM_SWITCH(sh.tl, {
+    const unsigned g = grid_for(m.n_nod, BS / TL, sh.cap);
+    hipLaunchKernelGGL((k_spmv<T, ND, TL, NRP>), dim3(g), dim3(BS), 0, s, m, val, x, y);
+  });
+  TFM_CHECK_LAUNCH();
+  return TFM_OK;
+}
+
+template <typename T, int ND, int NRP>
+static int spmv_dir_launch(hipStream_t s, const tfm_mesh_t &m, const T *val, const T *u, T *p,
+                           T *sv, int it, int seg_first, const double *part_in, int n_in,
+                           const PcgState &st, unsigned *grid_out) {
+  const SpmvShape sh = spmv_shape<T>(m, NRP);
+  unsigned g = 1;
+  TFM_TEAM_SWITCH(sh.tl, {
+    if (sh.scal) {
+      g = grid_for(m.n_nod, BS / TL, sh.cap);
+      TFM_REQUIRE((int64_t)g <= st.pq_cap, "SpMV grid exceeds the partial-sum buffer");
+      hipLaunchKernelGGL((k_pcg_spmv_dir<T, ND, TL, NRP, true>), dim3(g), dim3(BS), 0, s, m, val,
+                         u, p, sv, it, seg_first, part_in, n_in, st);
+    } else {
+      // grid-stride over the nodes with every block resident in one round;
+      // the 8-wave variant unless holding it to 64 VGPRs spills registers
+      static const auto kern = pick_w8((const void *)k_pcg_spmv_dir_w8<T, ND, TL, NRP, false>)
+                                   ? k_pcg_spmv_dir_w8<T, ND, TL, NRP, false>
+                                   : k_pcg_spmv_dir<T, ND, TL, NRP, false>;
+      static const int resident = resident_grid((const void *)kern);
+      g = grid_for(m.n_nod, BS / TL, std::min(sh.cap, resident));
+      TFM_REQUIRE((int64_t)g <= st.pq_cap, "SpMV grid exceeds the partial-sum buffer");
+      hipLaunchKernelGGL(kern, dim3(g), dim3(BS), 0, s, m, val, u, p, sv, it, seg_first, part_in,
+                         n_in, st);
+    }
+  });
+  TFM_CHECK_LAUNCH();
+  *grid_out = g;
+  return TFM_OK;
+}
+
 int launch_spmv(hipStream_t s, const tfm_mesh_t &m, int prec, const void *vals, const void *x,
                 void *y, int nrhs) {
   if (m.n_nod == 0) return TFM_OK;
   int rc = TFM_OK;
   TFM_ND_SWITCH(m.ndof, TFM_NRP_SWITCH(nrhs, {
     if (prec == 0)
-      rc = spmv_launch<double, ND, NRP>(s, m, (const double *)vals, (const double *)x, (double *)y,
-                                        nullptr, nullptr, nullptr);
+      rc = spmv_launch<double, ND, NRP>(s, m, (const double *)vals, (const double *)x, (double *)y);
     else
-      rc = spmv_launch<float, ND, NRP>(s, m, (const float *)vals, (const float *)x, (float *)y,
-                                       nullptr, nullptr, nullptr);
+      rc = spmv_launch<float, ND, NRP>(s, m, (const float *)vals, (const float *)x, (float *)y);
   }));
   return rc;
 }
@@ -590,19 +883,22 @@ struct tfm_pcg {
   int nrp = 1;    // capacity padded to a power of two
   int max_iter = 0;
   int device = 0;
-  // inner vectors, interleaved [n][nrp]; sized for doubles, used as float in mixed mode
-  double *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  // inner vectors, interleaved [n][nrp]; sized for doubles, used as float in
+  // mixed mode: x (correction), r (residual), u = M r, p (direction), s = K p
+  double *x = nullptr, *r = nullptr, *u = nullptr, *p = nullptr, *s = nullptr;
   double *minv = nullptr;                                  // sized n*3 doubles
   double *xm = nullptr, *r64 = nullptr, *f64 = nullptr;    // FP64 work vectors [n]
   double *work = nullptr;                                  // internal-force scratch
   int64_t work_len = 0;
-  double *scal = nullptr, *tol2 = nullptr;
+  double *scal = nullptr, *tol2 = nullptr, *rzold = nullptr;
   double *part_pq = nullptr, *part_upd = nullptr, *part_aux = nullptr, *part_res = nullptr;
-  double *res2 = nullptr, *rzold = nullptr;
+  PcgCoef *coef = nullptr;  // [2]: coefficients of A and B (scalar-kernel mode)
+  double *res2 = nullptr;
   int *done = nullptr, *alldone = nullptr;
+  int pq_cap = 0;                                          // rows of part_pq
   int *h_done = nullptr;                                   // pinned host mirrors
   double *h_res = nullptr;
-  // optional event timing of the inner SpMV launches
+  // optional event timing of the SpMV launches (kernel A) of the iterations
   bool timing = false;
   double spmv_ms = 0.0;
   int64_t spmv_count = 0;
@@ -619,17 +915,21 @@ static int pcg_alloc(tfm_pcg *h) {
   const int64_t n = h->n;
   const int nrp = h->nrp;
   const size_t vbytes = sizeof(double) * (size_t)n * nrp;
-  for (double **v : {&h->x, &h->r, &h->z, &h->p, &h->q}) TFM_CHECK_HIP(hipMalloc(v, vbytes));
+  for (double **v : {&h->x, &h->r, &h->u, &h->p, &h->s}) TFM_CHECK_HIP(hipMalloc(v, vbytes));
   TFM_CHECK_HIP(hipMalloc(&h->minv, sizeof(double) * (size_t)n * 3));
   for (double **v : {&h->xm, &h->r64, &h->f64}) TFM_CHECK_HIP(hipMalloc(v, sizeof(double) * (size_t)n));
   TFM_CHECK_HIP(hipMalloc(&h->scal, sizeof(double) * (size_t)(h->max_iter + 2) * nrp * NSCAL));
   TFM_CHECK_HIP(hipMalloc(&h->tol2, sizeof(double) * nrp));
-  TFM_CHECK_HIP(hipMalloc(&h->part_pq, sizeof(double) * RB * nrp));
-  TFM_CHECK_HIP(hipMalloc(&h->part_upd, sizeof(double) * RB * 2 * nrp));
-  TFM_CHECK_HIP(hipMalloc(&h->part_aux, sizeof(double) * RB * 3 * nrp));
-  TFM_CHECK_HIP(hipMalloc(&h->part_res, sizeof(double) * RB * 2));
-  TFM_CHECK_HIP(hipMalloc(&h->res2, sizeof(double) * 2));
   TFM_CHECK_HIP(hipMalloc(&h->rzold, sizeof(double) * nrp));
+  // one eta partial set per block of A; its widest grid has one node per
+  // 8-lane share of a block, at most n / 8 + 1 blocks
+  h->pq_cap = (int)std::max<int64_t>(RBA, n / 8 + 1);
+  TFM_CHECK_HIP(hipMalloc(&h->part_pq, sizeof(double) * (size_t)h->pq_cap * nrp));
+  TFM_CHECK_HIP(hipMalloc(&h->part_upd, sizeof(double) * RBV * 2 * nrp));
+  TFM_CHECK_HIP(hipMalloc(&h->part_aux, sizeof(double) * RBV * nrp));
+  TFM_CHECK_HIP(hipMalloc(&h->part_res, sizeof(double) * RBV * 2));
+  TFM_CHECK_HIP(hipMalloc(&h->coef, sizeof(PcgCoef) * 2));
+  TFM_CHECK_HIP(hipMalloc(&h->res2, sizeof(double) * 2));
   TFM_CHECK_HIP(hipMalloc(&h->done, sizeof(int) * nrp));
   TFM_CHECK_HIP(hipMalloc(&h->alldone, sizeof(int)));
   TFM_CHECK_HIP(hipHostMalloc(&h->h_done, sizeof(int) * (nrp + 1), hipHostMallocDefault));
@@ -639,10 +939,11 @@ static int pcg_alloc(tfm_pcg *h) {
 }
 
 static void pcg_free(tfm_pcg *h) {
-  for (double *ptr : {h->x, h->r, h->z, h->p, h->q, h->minv, h->xm, h->r64, h->f64, h->work,
-                      h->scal, h->tol2, h->part_pq, h->part_upd, h->part_aux, h->part_res,
-                      h->res2, h->rzold})
+  for (double *ptr : {h->x, h->r, h->u, h->p, h->s, h->minv, h->xm, h->r64, h->f64, h->work,
+                      h->scal, h->tol2, h->rzold, h->part_pq, h->part_upd, h->part_aux,
+                      h->part_res, h->res2})
     if (ptr) (void)hipFree(ptr);
+  if (h->coef) (void)hipFree(h->coef);
   if (h->done) (void)hipFree(h->done);
   if (h->alldone) (void)hipFree(h->alldone);
   if (h->h_done) (void)hipHostFree(h->h_done);
@@ -692,11 +993,17 @@ extern "C" int tfm_pcg_timing(tfm_pcg_t *h, int32_t enable, double *spmv_ms,
 
 namespace {
 
-// Operator callbacks of one solve: the inner SpMV (y = A x in the inner type,
-// with optional dot partials; returns the grid size through *g) and the FP64
-// true residual of rhs r (res = b - A x; returns ||res||^2 and ||b||^2).
+// Operator callbacks of one solve: kernel A of an iteration (reads n_in
+// partial sets of (gamma, rr) at part_in; returns its grid, the number of eta
+// partial sets, through *g), the plain inner SpMV y = K x, and the FP64 true
+// residual of one rhs (res = b - K x; returns ||res||^2 and ||b||^2). With
+// `scal`, one-block scalar kernels prepare the coefficients of A and B.
 template <typename T> struct Ops {
-  std::function<int(const T *, T *, const int *, double *, unsigned *)> spmv;
+  bool scal = false;
+  std::function<int(const T *, T *, T *, int, int, const double *, int, const PcgState &,
+                    unsigned *)>
+      spmv_dir;
+  std::function<int(const T *, T *)> spmv;
   std::function<int(const double *, const double *, double *, double *, double *)> residual;
 };
 
@@ -718,16 +1025,18 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
             int nrhs, double rtol, double delta, int32_t *iters, double *relres, int32_t *outer) {
   const bool fp64 = std::is_same<T, double>::value;
   const int64_t n = n_nod * ND;
-  T *xv = (T *)h->x, *rv = (T *)h->r, *zv = (T *)h->z, *pv = (T *)h->p, *qv = (T *)h->q;
+  T *xv = (T *)h->x, *rv = (T *)h->r, *uv = (T *)h->u, *pv = (T *)h->p, *sv = (T *)h->s;
   const T *minv = (const T *)h->minv;
-  PcgState st{h->scal, h->tol2, h->done, h->alldone, h->part_pq, h->part_upd, h->part_aux};
+  PcgState st{h->scal,     h->tol2,     h->rzold,    h->done,     h->alldone,
+              h->part_pq,  h->part_upd, h->part_aux, h->pq_cap,   h->coef,
+              h->coef + 1};
   const unsigned g1 = (unsigned)((n + 255) / 256);
-  const unsigned gv = grid_for(n_nod * NRP, BS);
+  const unsigned gv = grid_for(n_nod * NRP, BS, RBV);
   std::vector<double> htol(NRP, -1.0), bnorm(NRP, 0.0), rprev(NRP, 0.0), rnow(NRP, 0.0);
   std::vector<int> fin(NRP, 1), stall(NRP, 0);
   for (int r = 0; r < nrhs; ++r) iters[r] = 0;
 
-  // residuals of the initial guesses, staged as the inner residual columns
+  // true residual of rhs r at the current x, staged as the inner residual column
   auto stage = [&](int r, bool first) -> int {
     double rr2, bb2;
     int rc = ops.residual(b + (int64_t)r * n, x + (int64_t)r * n, h->r64, &rr2, &bb2);
@@ -750,27 +1059,38 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
     TFM_CHECK_LAUNCH();
     return TFM_OK;
   };
-  hipLaunchKernelGGL((k_fill_zero<T>), dim3((unsigned)((n * NRP + 255) / 256)), dim3(256), 0, s,
-                     n * NRP, xv);
-  hipLaunchKernelGGL((k_fill_zero<T>), dim3((unsigned)((n * NRP + 255) / 256)), dim3(256), 0, s,
-                     n * NRP, rv);
-  TFM_CHECK_LAUNCH();
+  // segment start: u = M r, then keep (cont) or reset (mask & ~cont) the directions
+  auto seg_start_kernels = [&](int it, int mask, int cont, const SegInfo &seg) -> int {
+    TFM_CHECK_HIP(hipMemcpyAsync(h->tol2, htol.data(), sizeof(double) * NRP, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((k_pcg_seg_dots<T, ND, NRP>), dim3(gv), dim3(BS), 0, s, n_nod,
+                       (const T *)rv, uv, (const T *)pv, minv, st);
+    TFM_CHECK_LAUNCH();
+    hipLaunchKernelGGL((k_pcg_seg_apply<T, ND, NRP>), dim3(gv), dim3(BS), 0, s, n_nod, it, mask,
+                       cont, seg, (const T *)uv, pv, sv, st, (int)gv);
+    TFM_CHECK_LAUNCH();
+    if (cont) return ops.spmv(pv, sv);
+    return TFM_OK;
+  };
+  for (T *v : {xv, rv, pv, sv}) {
+    hipLaunchKernelGGL((k_fill_zero<T>), dim3((unsigned)((n * NRP + 255) / 256)), dim3(256), 0, s,
+                       n * NRP, v);
+    TFM_CHECK_LAUNCH();
+  }
+  SegInfo seg{};
+  int mask = 0;
   for (int r = 0; r < nrhs; ++r) {
     const int rc = stage(r, true);
     if (rc != TFM_OK) return rc;
+    if (!fin[r]) mask |= 1 << r;
   }
-  TFM_CHECK_HIP(hipMemcpyAsync(h->tol2, htol.data(), sizeof(double) * NRP, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL((k_pcg_start<T, ND, NRP>), dim3(gv), dim3(BS), 0, s, n_nod, (const T *)rv, zv,
-                     pv, minv, st);
-  TFM_CHECK_LAUNCH();
-  hipLaunchKernelGGL((k_pcg_start_finish<NRP>), dim3(1), dim3(BS), 0, s, 0, (int)gv, st);
-  TFM_CHECK_LAUNCH();
+  {
+    const int rc = seg_start_kernels(0, mask, 0, seg);
+    if (rc != TFM_OK) return rc;
+  }
 
   const int CHK = 16;
-  int it = 0, n_outer = 0;
-  bool any_live = false;
-  for (int r = 0; r < nrhs; ++r) any_live |= !fin[r];
-  int seg_start = 0;
+  int it = 0, n_outer = 0, seg_first = 0;
+  bool any_live = mask != 0;
   while (any_live) {
     // run the current segment in batches of CHK iterations
     bool seg_done = false;
@@ -778,16 +1098,28 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
       const int start = it;
       const int stop = std::min(h->max_iter, it + CHK);
       for (; it < stop; ++it) {
+        if (ops.scal) {
+          hipLaunchKernelGGL((k_pcg_scal_dir<NRP>), dim3(1), dim3(BS), 0, s, it, seg_first,
+                             (const double *)h->part_upd, (int)gv, st);
+          TFM_CHECK_LAUNCH();
+        }
         if (h->timing) TFM_CHECK_HIP(hipEventRecord(h->ev[2 * (it - start)], s));
         unsigned npq = 0;
-        int rc = ops.spmv(pv, qv, h->alldone, h->part_pq, &npq);
+        int rc = ops.spmv_dir(uv, pv, sv, it, seg_first, h->part_upd, (int)gv, st, &npq);
         if (rc != TFM_OK) return rc;
         if (h->timing) TFM_CHECK_HIP(hipEventRecord(h->ev[2 * (it - start) + 1], s));
-        hipLaunchKernelGGL((k_pcg_update<T, ND, NRP>), dim3(gv), dim3(BS), 0, s, n_nod, it, xv, rv,
-                           zv, (const T *)pv, (const T *)qv, minv, st, (int)npq);
-        TFM_CHECK_LAUNCH();
-        hipLaunchKernelGGL((k_pcg_dir<T, ND, NRP>), dim3(gv), dim3(BS), 0, s, n_nod, it,
-                           (const T *)zv, pv, st, (int)gv);
+        if (ops.scal) {
+          hipLaunchKernelGGL((k_pcg_scal_alpha<NRP>), dim3(1), dim3(1024), 0, s, it,
+                             (const double *)h->part_pq, (int)npq, st);
+          TFM_CHECK_LAUNCH();
+          hipLaunchKernelGGL((k_pcg_update<T, ND, NRP, true>), dim3(gv), dim3(BS), 0, s, n_nod, it,
+                             xv, rv, uv, (const T *)pv, (const T *)sv, minv, st,
+                             (const double *)h->part_pq, (int)npq);
+        } else {
+          hipLaunchKernelGGL((k_pcg_update<T, ND, NRP, false>), dim3(gv), dim3(BS), 0, s, n_nod, it,
+                             xv, rv, uv, (const T *)pv, (const T *)sv, minv, st,
+                             (const double *)h->part_pq, (int)npq);
+        }
         TFM_CHECK_LAUNCH();
       }
       TFM_CHECK_HIP(hipMemcpyAsync(h->h_done, h->done, sizeof(int) * NRP, hipMemcpyDeviceToHost, s));
@@ -809,13 +1141,15 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
       }
     }
     for (int r = 0; r < nrhs; ++r)
-      if (!fin[r]) iters[r] += std::max(0, (h->h_done[r] >= 0 ? h->h_done[r] : it) - seg_start);
+      if (!fin[r]) iters[r] += std::max(0, (h->h_done[r] >= 0 ? h->h_done[r] : it) - seg_first);
     // fold the corrections into x and replace the residuals by true ones;
     // FP64 (whose corrections only absorb the drift of the recursive residual)
     // and a mixed solve whose true residual stopped falling restart from the
-    // new residual, other mixed solves continue their recurrence
-    int mask = 0, restart = 0;
+    // new residual, other mixed solves keep their direction
+    int cont = 0;
+    mask = 0;
     any_live = false;
+    for (int r = 0; r < NRP; ++r) seg.d[r] = h->h_done[r];
     for (int r = 0; r < nrhs; ++r) {
       if (fin[r]) continue;
       hipLaunchKernelGGL((k_take_column<T, NRP>), dim3(g1), dim3(256), 0, s, n, r, xv,
@@ -827,7 +1161,7 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
         mask |= 1 << r;
         const bool stuck = rnow[r] > 0.9 * rprev[r];
         stall[r] = stuck ? stall[r] + 1 : 0;
-        if (fp64 || stuck) restart |= 1 << r;
+        if (!fp64 && !stuck && seg.d[r] > seg_first) cont |= 1 << r;
         any_live = true;
       }
     }
@@ -841,17 +1175,9 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
       }
     ++n_outer;
     if (!any_live || it >= h->max_iter) break;
-    TFM_CHECK_HIP(hipMemcpyAsync(h->tol2, htol.data(), sizeof(double) * NRP, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL((k_reliable_dots<T, ND, NRP>), dim3(gv), dim3(BS), 0, s, n_nod,
-                       (const T *)rv, zv, (const T *)pv, minv, st);
-    TFM_CHECK_LAUNCH();
-    hipLaunchKernelGGL((k_reliable_prep<NRP>), dim3(1), dim3(64), 0, s, mask, restart, seg_start,
-                       st, h->rzold);
-    TFM_CHECK_LAUNCH();
-    hipLaunchKernelGGL((k_reliable_apply<T, ND, NRP>), dim3(gv), dim3(BS), 0, s, n_nod, it, mask,
-                       (const T *)zv, pv, st, (const double *)h->rzold, (int)gv);
-    TFM_CHECK_LAUNCH();
-    seg_start = it;
+    const int rc = seg_start_kernels(it, mask, cont, seg);
+    if (rc != TFM_OK) return rc;
+    seg_first = it;
   }
   if (outer) *outer = n_outer;
   for (int r = 0; r < nrhs; ++r)
@@ -898,7 +1224,7 @@ int fem_residual(tfm_pcg *h, hipStream_t s, const tfm_system_t &sys, const doubl
     if (rc != TFM_OK) return rc;
     con = sys.con;
   }
-  const unsigned gr = grid_for(n, BS);
+  const unsigned gr = grid_for(n, BS, RBV);
   hipLaunchKernelGGL(k_residual, dim3(gr), dim3(BS), 0, s, n, con, b, x, (const double *)h->f64,
                      res, h->part_res);
   TFM_CHECK_LAUNCH();
@@ -935,8 +1261,14 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
       rc = build_precond<double>(s, m.n_nod, ND, sys->diag_blocks, pc, (double *)h->minv);
       if (rc != TFM_OK) return rc;
       Ops<double> ops;
-      ops.spmv = [&](const double *xi, double *yo, const int *ad, double *part, unsigned *g) -> int {
-        return spmv_launch<double, ND, NRP>(s, m, sys->val64, xi, yo, ad, part, g);
+      ops.scal = spmv_shape<double>(m, NRP).scal;
+      ops.spmv_dir = [&](const double *u, double *p, double *sv, int it, int sf,
+                         const double *pin, int n_in, const PcgState &st, unsigned *g) -> int {
+        return spmv_dir_launch<double, ND, NRP>(s, m, sys->val64, u, p, sv, it, sf, pin, n_in, st,
+                                                g);
+      };
+      ops.spmv = [&](const double *xi, double *yo) -> int {
+        return spmv_launch<double, ND, NRP>(s, m, sys->val64, xi, yo);
       };
       ops.residual = residual;
       rc = run_pcg<double, ND, NRP>(h, s, m.n_nod, ops, b, x, nrhs, rtol, inner_rtol, iters,
@@ -945,8 +1277,14 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
       rc = build_precond<float>(s, m.n_nod, ND, sys->diag_blocks, pc, (float *)h->minv);
       if (rc != TFM_OK) return rc;
       Ops<float> ops;
-      ops.spmv = [&](const float *xi, float *yo, const int *ad, double *part, unsigned *g) -> int {
-        return spmv_launch<float, ND, NRP>(s, m, sys->val32, xi, yo, ad, part, g);
+      ops.scal = spmv_shape<float>(m, NRP).scal;
+      ops.spmv_dir = [&](const float *u, float *p, float *sv, int it, int sf, const double *pin,
+                         int n_in, const PcgState &st, unsigned *g) -> int {
+        return spmv_dir_launch<float, ND, NRP>(s, m, sys->val32, u, p, sv, it, sf, pin, n_in, st,
+                                               g);
+      };
+      ops.spmv = [&](const float *xi, float *yo) -> int {
+        return spmv_launch<float, ND, NRP>(s, m, sys->val32, xi, yo);
       };
       ops.residual = residual;
       rc = run_pcg<float, ND, NRP>(h, s, m.n_nod, ops, b, x, nrhs, rtol, inner_rtol, iters,
@@ -972,14 +1310,13 @@ __global__ void k_csr_diag(int64_t n, const int64_t *rowptr, const int32_t *col,
   d[i] = v;
 }
 
+constexpr int CSR_TL = 8;
+
 int csr_spmv(hipStream_t s, int64_t n, const int64_t *rowptr, const int32_t *col,
-             const double *val, const double *x, double *y, const int *alldone, double *part,
-             unsigned *grid_out) {
-  const unsigned g = grid_for(n, BS / 8);
-  hipLaunchKernelGGL((k_spmv_csr<8>), dim3(g), dim3(BS), 0, s, n, rowptr, col, val, x, y,
-                     alldone, part);
+             const double *val, const double *x, double *y) {
+  const unsigned g = grid_for(n, BS / CSR_TL, RBA);
+  hipLaunchKernelGGL((k_spmv_csr<CSR_TL>), dim3(g), dim3(BS), 0, s, n, rowptr, col, val, x, y);
   TFM_CHECK_LAUNCH();
-  if (grid_out) *grid_out = g;
   return TFM_OK;
 }
 
@@ -990,7 +1327,7 @@ extern "C" int tfm_csr_spmv(tfm_stream_t s, int64_t n, const int64_t *rowptr, co
   TFM_REQUIRE(n >= 0, "negative size");
   if (n == 0) return TFM_OK;
   TFM_REQUIRE(rowptr && col && val && x && y, "null argument");
-  return csr_spmv((hipStream_t)s, n, rowptr, col, val, x, y, nullptr, nullptr, nullptr);
+  return csr_spmv((hipStream_t)s, n, rowptr, col, val, x, y);
 }
 
 extern "C" int tfm_csr_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, int64_t n,
@@ -1005,16 +1342,25 @@ extern "C" int tfm_csr_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, int64_t n,
   hipLaunchKernelGGL(k_csr_diag, dim3(g1), dim3(256), 0, s, n, rowptr, col, val, h->f64);
   TFM_CHECK_LAUNCH();
   hipLaunchKernelGGL((k_precond<double, 1>), dim3(g1), dim3(256), 0, s, n, (const double *)h->f64,
-                     (int)TFM_PC_JACOBI, h->minv);
+                     (int)TFM_PC_JACOBI, (double *)h->minv);
   TFM_CHECK_LAUNCH();
   Ops<double> ops;
-  ops.spmv = [&](const double *xi, double *yo, const int *ad, double *part, unsigned *g) -> int {
-    return csr_spmv(s, n, rowptr, col, val, xi, yo, ad, part, g);
+  ops.spmv_dir = [&](const double *u, double *p, double *sv, int it, int sf, const double *pin,
+                     int n_in, const PcgState &st, unsigned *g) -> int {
+    *g = grid_for(n, BS / CSR_TL, RBA);
+    TFM_REQUIRE((int64_t)*g <= st.pq_cap, "SpMV grid exceeds the partial-sum buffer");
+    hipLaunchKernelGGL((k_pcg_csr_dir<CSR_TL>), dim3(*g), dim3(BS), 0, s, n, rowptr, col, val, u,
+                       p, sv, it, sf, pin, n_in, st);
+    TFM_CHECK_LAUNCH();
+    return TFM_OK;
+  };
+  ops.spmv = [&](const double *xi, double *yo) -> int {
+    return csr_spmv(s, n, rowptr, col, val, xi, yo);
   };
   ops.residual = [&](const double *bb, const double *xx, double *res, double *rr, double *b2) -> int {
-    int rc = csr_spmv(s, n, rowptr, col, val, xx, h->f64, nullptr, nullptr, nullptr);
+    int rc = csr_spmv(s, n, rowptr, col, val, xx, h->f64);
     if (rc != TFM_OK) return rc;
-    const unsigned gr = grid_for(n, BS);
+    const unsigned gr = grid_for(n, BS, RBV);
     hipLaunchKernelGGL(k_residual, dim3(gr), dim3(BS), 0, s, n, (const uint8_t *)nullptr, bb, xx,
                        (const double *)h->f64, res, h->part_res);
     TFM_CHECK_LAUNCH();

@@ -114,55 +114,76 @@ class _StaticSolve(torch.autograd.Function):
     """u = K(p)^{-1} [F_ext on free DOFs; U_D on constrained DOFs], with the
     implicit-function backward of NewtonRaphsonAdjoint (sparse.py:641-724):
     K^T lam = grad_u (free part), dL/dp = -lam^T d(K u)/dp, dL/dF = lam,
-    dL/dU_D = grad_u - (K lam) on constrained DOFs."""
+    dL/dU_D = grad_u - (K lam) on constrained DOFs.
+
+    F_ext may hold several load cases ([nrhs, n_dofs]); they are solved as one
+    batched multi-RHS PCG, and so are their adjoints."""
 
     @staticmethod
     def forward(ctx, fem, cfg, F_ext, U_D, p0, p1, scale):
+        n = fem.n_dofs
+        F2 = F_ext.reshape(-1, n)
+        nr = F2.shape[0]
+        U2 = U_D.reshape(-1, n).expand(nr, n)
         system = _System(fem, cfg["kind"], p0, p1, scale, cfg["precision"])
         con = system.con.bool()
-        u0 = torch.where(con, U_D, torch.zeros_like(U_D))
-        d = fem._newton(system, cfg, F_ext, u0, con)
+        u0 = torch.where(con, U2, torch.zeros_like(U2))
+        d = fem._newton(system, cfg, F2, u0, con)
         u = u0 + d
         ctx.fem, ctx.cfg, ctx.system = fem, cfg, system
+        ctx.shapes = (F_ext.shape, U_D.shape)
         ctx.save_for_backward(u)
         fem._last_system = system
-        return u
+        return u.reshape(F_ext.shape)
 
     @staticmethod
     def backward(ctx, g):
         (u,) = ctx.saved_tensors
         fem, cfg, system = ctx.fem, ctx.cfg, ctx.system
+        n = fem.n_dofs
         con = system.con.bool()
-        g = g.contiguous()
-        g_free = torch.where(con, torch.zeros_like(g), g)
+        g2 = g.reshape(-1, n).contiguous()
+        g_free = torch.where(con, torch.zeros_like(g2), g2)
         t0 = time.perf_counter()
-        lam = fem._pcg(system, g_free.unsqueeze(0), cfg, warm=cfg.get("x0_adjoint"))[0]
+        warm = cfg.get("x0_adjoint")
+        if warm is not None and warm.numel() != g_free.numel():
+            warm = None
+        lam = fem._pcg(system, g_free, cfg, warm=warm)
         if cfg.get("cached") is not None and cfg.get("update_cache"):
-            cfg["cached"].update_grad(lam)
+            cfg["cached"].update_grad(lam.reshape(g.shape))
         fem.last_solve_info["adjoint"] = dict(fem._last_pcg, time=time.perf_counter() - t0)
         needs = ctx.needs_input_grad
-        gF = lam if needs[2] else None
+        F_shape, D_shape = ctx.shapes
+        gF = lam.reshape(F_shape) if needs[2] else None
         gD = None
         if needs[3]:
-            Klam = system.force(lam)
-            gD = torch.where(con, g - Klam, torch.zeros_like(g))
+            Klam = torch.stack([system.force(row) for row in lam])
+            gD = torch.where(con, g2 - Klam, torch.zeros_like(g2))
+            gD = gD.sum(0).reshape(D_shape) if gD.shape[0] != 1 and len(D_shape) == 1 else gD.reshape(D_shape)
         g0 = g1 = gs = None
         if any(needs[4:7]):
-            out = system.sensitivity(u, lam, -1.0)[0]
+            out = system.sensitivity(u, lam, -1.0).sum(0)
             g0, g1, gs = system.param_grads(out, needs[4:7])
         return None, None, gF, gD, g0, g1, gs
 
 
 class _Integrate(torch.autograd.Function):
     """Converged-state evaluation (base.py:844-867): internal forces f = K u
-    (reactions included), Gauss-point mean gradient and flux."""
+    (reactions included), Gauss-point mean gradient and flux, for one state
+    [n_dofs] or several [nrhs, n_dofs]."""
 
     @staticmethod
     def forward(ctx, fem, kind, u, p0, p1, scale):
-        mat_holder = _MatHolder(kind, p0, p1, scale)
-        F, _, gm, qm, _, _ = fem._forces(mat_holder.mat, u, means=True)
-        ctx.fem, ctx.holder = fem, mat_holder
+        h = _MatHolder(kind, p0, p1, scale)
+        rows = u.reshape(-1, fem.n_dofs)
+        outs = [fem._forces(h.mat, row, means=True) for row in rows]
+        F = torch.stack([o[0] for o in outs])
+        gm = torch.stack([o[2] for o in outs])
+        qm = torch.stack([o[3] for o in outs])
+        ctx.fem, ctx.holder, ctx.single = fem, h, u.dim() == 1
         ctx.save_for_backward(u)
+        if u.dim() == 1:
+            return F[0], gm[0], qm[0]
         return F, gm, qm
 
     @staticmethod
@@ -180,12 +201,12 @@ class _Integrate(torch.autograd.Function):
         gu = g0 = g1 = gs = None
         if gF is None:
             return None, None, gu, g0, g1, gs
-        gF = gF.contiguous()
+        gF2 = gF.reshape(-1, fem.n_dofs).contiguous()
         if needs[2]:
-            gu = fem._forces(h.mat, gF)[0]
+            gu = torch.stack([fem._forces(h.mat, row)[0] for row in gF2]).reshape(u.shape)
         if any(needs[3:6]):
             sysv = _SensView(fem, h)
-            out = sysv.sensitivity(u, gF, 1.0)[0]
+            out = sysv.sensitivity(u.reshape(-1, fem.n_dofs), gF2, 1.0).sum(0)
             g0, g1, gs = sysv.param_grads(out, needs[3:6])
         return None, None, gu, g0, g1, gs
 
@@ -411,29 +432,39 @@ class FEM(ABC):
 
     def _newton(self, system: _System, cfg: dict, F_ext: Tensor, u0: Tensor, con: Tensor) -> Tensor:
         """Newton loop of NewtonRaphsonAdjoint.forward (sparse.py:541-639) on the
-        correction d (zero on constrained DOFs) of a linear model."""
+        corrections d (zero on constrained DOFs) of a linear model, for one or
+        several load cases (rows of F_ext and u0) at once."""
         t0 = time.perf_counter()
         zero = torch.zeros_like(F_ext)
-        r = torch.where(con, zero, F_ext - system.force(u0))
-        res0 = float(torch.linalg.norm(r))
+
+        def residual(d):
+            Ku = torch.stack([system.force(row) for row in u0 + d])
+            return torch.where(con, zero, F_ext - Ku)
+
         d = torch.zeros_like(F_ext)
+        r = residual(d)
+        res0 = torch.linalg.norm(r, dim=1)
         iters = []
         for i in range(cfg["max_iter"]):
-            res = res0 if i == 0 else float(torch.linalg.norm(r))
-            if not math.isfinite(res):
+            res = torch.linalg.norm(r, dim=1) if i else res0
+            if not bool(torch.isfinite(res).all()):
                 break
-            if res < cfg["rtol"] * res0 or res < cfg["atol"]:
+            ok = (res < cfg["rtol"] * res0) | (res < cfg["atol"])
+            if bool(ok.all()):
                 self.last_solve_info["forward"] = dict(
-                    newton_iterations=i, pcg=iters, res0=res0, res=res,
+                    newton_iterations=i, pcg=iters, res0=res0.tolist(), res=res.tolist(),
                     time=time.perf_counter() - t0)
                 return d
+            rhs = torch.where(ok[:, None], zero, r)  # converged cases get a zero rhs
             warm = cfg.get("x0") if i == 0 else None
-            dd = self._pcg(system, r.unsqueeze(0), cfg, warm=warm)[0]
+            if warm is not None and warm.numel() != rhs.numel():
+                warm = None
+            dd = self._pcg(system, rhs, cfg, warm=warm)
             iters.append(self._last_pcg)
             if i == 0 and cfg.get("cached") is not None and cfg.get("update_cache"):
-                cfg["cached"].update_x(dd)
+                cfg["cached"].update_x(dd.reshape(-1) if dd.shape[0] == 1 else dd)
             d = d + dd
-            r = torch.where(con, zero, F_ext - system.force(u0 + d))
+            r = residual(d)
         raise RuntimeError("Newton-Raphson iteration did not converge.")
 
     # ------------------------------------------------------- API methods
@@ -730,6 +761,59 @@ class FEM(ABC):
         if return_intermediate:
             return u_all, f_all, q_all, g_all, s_all
         return u_all[-1], f_all[-1], q_all[-1], g_all[-1], s_all[-1]
+
+    def solve_load_cases(
+        self,
+        forces: Tensor,
+        rtol: float = 1e-8,
+        atol: float = 1e-6,
+        stol: float = 1e-10,
+        max_iter: int = 10,
+        differentiable_parameters: Tensor | Iterable[Tensor] | None = None,
+        precision: str | None = None,
+        preconditioner: str = "block_jacobi",
+        inner_rtol: float = 0.1,
+    ) -> tuple[Tensor, Tensor]:
+        """Solves several load cases of one model at once (no counterpart in the
+        reference, whose solve() takes one load state): `forces` [nrhs, n_nod,
+        n_dof] with the model's own prescribed values and constraints. The
+        cases share one assembled matrix and one batched multi-RHS PCG; their
+        adjoints are solved together the same way. Returns (u, f), each
+        [nrhs, n_nod, n_dof]."""
+        if self.n_state != 0:
+            raise NotImplementedError("stateful materials are not part of this build")
+        forces = torch.as_tensor(forces).to(self.device)
+        if forces.dim() != 3 or forces.shape[1:] != (self.n_nod, self.n_dof_per_node):
+            raise ValueError("forces must have shape [n_cases, n_nod, n_dof_per_node]")
+        if not 1 <= forces.shape[0] <= 16:
+            raise ValueError("between 1 and 16 load cases can be solved at once")
+        precision = precision or default_precision()
+        if differentiable_parameters is None:
+            params = ()
+        elif isinstance(differentiable_parameters, Tensor):
+            params = (differentiable_parameters,)
+        else:
+            params = tuple(differentiable_parameters)
+        track = any(p.requires_grad for p in params)
+        kind, p0, p1, scale = self._material_params()
+        cfg = dict(kind=kind, precision=precision, pc=preconditioner, stol=stol, rtol=rtol,
+                   atol=atol, max_iter=max_iter, inner_rtol=inner_rtol)
+        self.last_solve_info = {}
+        self._K_cache = None
+        F = forces.reshape(forces.shape[0], -1).to(torch.float64)
+        D = self._dirichlet.reshape(-1)
+        if not track:
+            F, D, p0 = F.detach(), D.detach(), p0.detach()
+            p1 = p1.detach() if p1 is not None else None
+            scale = scale.detach() if scale is not None else None
+        with torch.set_grad_enabled(track):
+            u = _StaticSolve.apply(self, cfg, F, D, p0, p1, scale)
+            f, _, _ = _Integrate.apply(self, kind, u, p0, p1, scale)
+        shape = (forces.shape[0], self.n_nod, self.n_dof_per_node)
+        u, f = u.reshape(shape), f.reshape(shape)
+        if not track:
+            u, f = u.detach(), f.detach()
+        return u, f
 
     def _per_gp_outputs(self, u_all: Tensor, init: Tensor):
         kind, p0, p1, scale = self._material_params()

@@ -227,13 +227,17 @@ int tfm_pcg_destroy(tfm_pcg_t *h);
  * prec: TFM_PREC_FP64 or TFM_PREC_MIXED; pc: TFM_PC_*. Stops when
  * ||b - K x||_2 <= rtol ||b||_2 for every right-hand side (the criterion of
  * the reference's CG/MINRES with rtol = stol, sparse.py:406-421).
+ * Mixed mode: each FP32 segment reduces the residual by inner_rtol before the
+ * next FP64 refinement, and the segment that can end the solve aims at
+ * final_rtol ||b||_2 (0 < final_rtol <= rtol; a margin below rtol for the error
+ * of the solution); FP64 mode aims at 0.7 rtol.
  * iters[nrhs] receives the total inner iterations, relres[nrhs] the final FP64
  * relative residual, outer[0] the refinement steps (mixed mode).
  * Returns TFM_ERR_NOCONV when max_iter is exhausted. */
 int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t s, const tfm_system_t *sys,
                   int32_t prec, int32_t pc, const double *b, double *x,
-                  int32_t nrhs, double rtol, double inner_rtol, int32_t *iters,
-                  double *relres, int32_t *outer);
+                  int32_t nrhs, double rtol, double inner_rtol, double final_rtol,
+                  int32_t *iters, double *relres, int32_t *outer);
 
 /* Event timing of the inner SpMV launches of tfm_pcg_solve on the solve's
  * stream: returns the accumulated milliseconds and launch count since the

@@ -1022,7 +1022,8 @@ int sum_residual(tfm_pcg *h, hipStream_t s, unsigned nblk, double *rr, double *b
 // next reliable update (mixed).
 template <typename T, int ND, int NRP>
 int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double *b, double *x,
-            int nrhs, double rtol, double delta, int32_t *iters, double *relres, int32_t *outer) {
+            int nrhs, double rtol, double delta, double final_rtol, int32_t *iters,
+            double *relres, int32_t *outer) {
   const bool fp64 = std::is_same<T, double>::value;
   const int64_t n = n_nod * ND;
   T *xv = (T *)h->x, *rv = (T *)h->r, *uv = (T *)h->u, *pv = (T *)h->p, *sv = (T *)h->s;
@@ -1051,8 +1052,12 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
     rnow[r] = rn;
     relres[r] = bnorm[r] > 0 ? rn / bnorm[r] : rn;
     fin[r] = rn <= rtol * bnorm[r];
-    const double target = (fp64 ? 0.7 : 0.5) * rtol * bnorm[r];
-    const double seg = fp64 ? target : std::max(delta * rn, target);
+    // the last segment stops below rtol: FP64 by a margin for the drift of
+    // its recursive residual, mixed at the caller's final_rtol
+    const double target = (fp64 ? 0.7 * rtol : final_rtol) * bnorm[r];
+    // a mixed segment reduces the residual by delta, and the one that can
+    // end the solve aims at the final target
+    const double seg = fp64 || delta * rn <= rtol * bnorm[r] ? target : delta * rn;
     htol[r] = fin[r] ? -1.0 : seg * seg;
     hipLaunchKernelGGL((k_set_column<T, NRP>), dim3(g1), dim3(256), 0, s, n, r,
                        (const double *)h->r64, rv, fin[r] ? 1 : 0);
@@ -1235,8 +1240,8 @@ int fem_residual(tfm_pcg *h, hipStream_t s, const tfm_system_t &sys, const doubl
 
 extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system_t *sys,
                              int32_t prec, int32_t pc, const double *b, double *x, int32_t nrhs,
-                             double rtol, double inner_rtol, int32_t *iters, double *relres,
-                             int32_t *outer) {
+                             double rtol, double inner_rtol, double final_rtol,
+                             int32_t *iters, double *relres, int32_t *outer) {
   TFM_REQUIRE(h && sys && sys->mesh && b && x && iters && relres, "null argument");
   const tfm_mesh_t &m = *sys->mesh;
   const int64_t n = m.n_nod * m.ndof;
@@ -1249,6 +1254,7 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
               "matrix values for the requested precision are missing");
   TFM_REQUIRE(rtol > 0.0, "rtol must be positive");
   TFM_REQUIRE(inner_rtol > 0.0 && inner_rtol < 1.0, "inner_rtol must be in (0, 1)");
+  TFM_REQUIRE(final_rtol > 0.0 && final_rtol <= rtol, "final_rtol must be in (0, rtol]");
   if (outer) *outer = 0;
   hipStream_t s = (hipStream_t)stream;
   const int nrp = pow2_cap(nrhs);
@@ -1271,8 +1277,8 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
         return spmv_launch<double, ND, NRP>(s, m, sys->val64, xi, yo);
       };
       ops.residual = residual;
-      rc = run_pcg<double, ND, NRP>(h, s, m.n_nod, ops, b, x, nrhs, rtol, inner_rtol, iters,
-                                    relres, outer);
+      rc = run_pcg<double, ND, NRP>(h, s, m.n_nod, ops, b, x, nrhs, rtol, inner_rtol, final_rtol,
+                                    iters, relres, outer);
     } else {
       rc = build_precond<float>(s, m.n_nod, ND, sys->diag_blocks, pc, (float *)h->minv);
       if (rc != TFM_OK) return rc;
@@ -1287,8 +1293,8 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
         return spmv_launch<float, ND, NRP>(s, m, sys->val32, xi, yo);
       };
       ops.residual = residual;
-      rc = run_pcg<float, ND, NRP>(h, s, m.n_nod, ops, b, x, nrhs, rtol, inner_rtol, iters,
-                                   relres, outer);
+      rc = run_pcg<float, ND, NRP>(h, s, m.n_nod, ops, b, x, nrhs, rtol, inner_rtol, final_rtol,
+                                   iters, relres, outer);
     }
   }));
   return rc;
@@ -1367,5 +1373,6 @@ extern "C" int tfm_csr_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, int64_t n,
     return sum_residual(h, s, gr, rr, b2);
   };
   int32_t outer = 0;
-  return run_pcg<double, 1, 1>(h, s, n, ops, b, x, 1, rtol, 0.5, iters, relres, &outer);
+  return run_pcg<double, 1, 1>(h, s, n, ops, b, x, 1, rtol, 0.5, 0.7 * rtol, iters, relres,
+                               &outer);
 }

@@ -122,6 +122,7 @@ _SIGNATURES: dict[str, tuple[list[Any], Any]] = {
             I32,
             F64,
             F64,
+            F64,
             C.POINTER(I32),
             C.POINTER(F64),
             C.POINTER(I32),

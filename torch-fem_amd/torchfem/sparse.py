@@ -120,8 +120,11 @@ class PCGSolver:
         return float(ms.value), int(cnt.value)
 
     def solve_system(self, sys: L.System, b: Tensor, x: Tensor, precision="mixed",
-                     preconditioner="block_jacobi", rtol=1e-10, inner_rtol=0.1) -> dict:
-        """K_bc x = b for the rows of b [nrhs, n] (x in/out), on an assembled system."""
+                     preconditioner="block_jacobi", rtol=1e-10, inner_rtol=0.1,
+                     final_rtol=None) -> dict:
+        """K_bc x = b for the rows of b [nrhs, n] (x in/out), on an assembled system.
+        Mixed precision: the last FP32 segment aims at final_rtol (default
+        0.7 rtol) so that the FP64 residual ends below rtol with a margin."""
         nrhs = b.shape[0]
         self._ensure(nrhs)
         its = (L.I32 * nrhs)()
@@ -132,6 +135,7 @@ class PCGSolver:
         t0 = time.perf_counter()
         rc = L.lib().tfm_pcg_solve(self._h, L.stream_of(self.device), C.byref(sys), prec, pc,
                                    L.ptr(b), L.ptr(x), nrhs, float(rtol), float(inner_rtol),
+                                   float(0.7 * rtol if final_rtol is None else final_rtol),
                                    its, rel, C.byref(outer))
         info = dict(iterations=list(its), relres=list(rel), outer=int(outer.value),
                     time=time.perf_counter() - t0, precision=precision, preconditioner=preconditioner)

@@ -31,7 +31,8 @@ namespace tfm {
 
 constexpr int BS = 256;   // threads per block of the solver kernels
 constexpr int RBA = 2048; // max blocks of the SpMV kernels (= eta partials per rhs)
-constexpr int RBV = 512;  // max blocks of the vector kernels (= partials per reduction)
+constexpr int RBV = 512;  // max blocks of the vector kernels whose partials every block of A sums
+constexpr int RBW = 4096; // max blocks of the vector kernels when a scalar kernel sums them
 constexpr int MAXR = 16;  // right-hand sides per solve
 constexpr int NSCAL = 4;  // per-iteration scalars: rz, pq, rr, spare
 enum { S_RZ = 0, S_PQ = 1, S_RR = 2 };
@@ -301,8 +302,8 @@ struct PcgState {
   int *done;         // [NRP] iteration at which rhs r stopped, -1 while running
   int *alldone;      // 1 once every rhs stopped
   double *part_pq;   // [pq_cap * NRP]   eta partials (A)
-  double *part_upd;  // [RBV * 2 * NRP]  (gamma, rr) partials (B and the segment start)
-  double *part_aux;  // [RBV * NRP]      r.p partials (segment start)
+  double *part_upd;  // [RBW * 2 * NRP]  (gamma, rr) partials (B and the segment start)
+  double *part_aux;  // [RBW * NRP]      r.p partials (segment start)
   int pq_cap;        // blocks of A whose partials part_pq can hold
   PcgCoef *coef_a;   // beta of A (scalar-kernel mode)
   PcgCoef *coef_b;   // alpha of B (scalar-kernel mode)
@@ -704,6 +705,83 @@ __global__ void k_mask(int64_t n, const uint8_t *con, const double *x, double *x
   if (i < n) xm[i] = con[i] ? 0.0 : x[i];
 }
 
+// Refinement residual res = b - K x with the row sums of K x carried in
+// double-double (an exact product by FMA, an exact sum by two-sum, per
+// term): the residual then has no FP64 rounding floor of the size
+// eps * |K| |x|, which near a tight tolerance would stall the refinement.
+// One team of TL lanes per node; partials of (res.res, b.b) per block.
+__device__ __forceinline__ void dd_add(double &hi, double &lo, double a, double b2) {
+  // (hi, lo) += (a, b2)
+  const double s = hi + a;
+  const double v = s - hi;
+  const double e = (hi - (s - v)) + (a - v) + lo + b2;
+  hi = s + e;
+  lo = e - (hi - s);
+}
+
+template <int ND, int TL>
+__global__ void __launch_bounds__(BS) k_residual_dd(tfm_mesh_t m, const double *__restrict__ val,
+                                                    const double *__restrict__ b,
+                                                    const double *__restrict__ x,
+                                                    double *__restrict__ res, double *part) {
+  __shared__ double sh[2 * BS];
+  constexpr int TPB = BS / TL;
+  const int lane = threadIdx.x % TL, team = threadIdx.x / TL;
+  double acc2[2] = {0.0, 0.0};
+  for (int64_t n = (int64_t)blockIdx.x * TPB + team; n < m.n_nod; n += (int64_t)gridDim.x * TPB) {
+    double hi[ND], lo[ND];
+#pragma unroll
+    for (int a = 0; a < ND; ++a) hi[a] = lo[a] = 0.0;
+    const int a0 = m.adj_ptr[n];
+    const int deg = m.adj_ptr[n + 1] - a0;
+    const double *vn = val + m.val_ptr[n];
+    auto block = [&](const Vec<double, ND> (&rows)[ND], const Vec<double, ND> &xv) {
+#pragma unroll
+      for (int a = 0; a < ND; ++a)
+#pragma unroll
+        for (int c = 0; c < ND; ++c) {
+          const double ph = rows[a].v[c] * xv.v[c];
+          dd_add(hi[a], lo[a], ph, fma(rows[a].v[c], xv.v[c], -ph));
+        }
+    };
+    if (deg > 0) {
+      const int L = ND * deg;
+      for (int p = lane; p < deg; p += TL) {
+        Vec<double, ND> rows[ND];
+#pragma unroll
+        for (int a = 0; a < ND; ++a) rows[a] = ldv<double, ND>(vn + a * L + p * ND);
+        block(rows, ldv<double, ND>(x + (int64_t)m.adj[a0 + p] * ND));
+      }
+    } else if (lane < ND) {
+#pragma unroll
+      for (int a = 0; a < ND; ++a)
+        if (a == lane) {
+          const double ph = vn[a] * x[n * ND + a];
+          dd_add(hi[a], lo[a], ph, fma(vn[a], x[n * ND + a], -ph));
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < TL; off <<= 1)
+#pragma unroll
+      for (int a = 0; a < ND; ++a) {
+        const double oh = __shfl_xor(hi[a], off, TL), ol = __shfl_xor(lo[a], off, TL);
+        dd_add(hi[a], lo[a], oh, ol);
+      }
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+      if (a == lane) {
+        const int64_t i = n * ND + a;
+        double rh = b[i], rl = 0.0;
+        dd_add(rh, rl, -hi[a], -lo[a]);
+        const double r = rh + rl;
+        res[i] = r;
+        acc2[0] += r * r;
+        acc2[1] += b[i] * b[i];
+      }
+  }
+  write_block_partials<2, 1>(acc2, sh, part);
+}
+
 __global__ void __launch_bounds__(BS) k_residual(int64_t n, const uint8_t *con, const double *b,
                                                  const double *x, const double *f, double *res,
                                                  double *part) {
@@ -718,11 +796,26 @@ __global__ void __launch_bounds__(BS) k_residual(int64_t n, const uint8_t *con, 
   write_block_partials<2, 1>(acc, sh, part);
 }
 
-__global__ void __launch_bounds__(BS) k_sum2(const double *part, int nblk, double *out) {
-  __shared__ double sh[BS];
-  __shared__ double tot[2];
-  sum_partials<2>(part, nblk, sh, tot);
-  if (threadIdx.x < 2) out[threadIdx.x] = tot[threadIdx.x];
+__global__ void __launch_bounds__(1024) k_sum2(const double *part, int nblk, double *out) {
+  // nblk partial pairs (block order), 512 pairs of accumulators
+  __shared__ double sh[1024];
+  const int tid = threadIdx.x, slot = tid % 2, c = tid / 2;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int b = c;
+  for (; b + 3 * 512 < nblk; b += 4 * 512) {
+    a0 += part[(size_t)b * 2 + slot];
+    a1 += part[(size_t)(b + 512) * 2 + slot];
+    a2 += part[(size_t)(b + 1024) * 2 + slot];
+    a3 += part[(size_t)(b + 1536) * 2 + slot];
+  }
+  for (; b < nblk; b += 512) a0 += part[(size_t)b * 2 + slot];
+  sh[tid] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  for (int w = 256; w >= 1; w >>= 1) {
+    if (tid < 2 * w) sh[tid] += sh[tid + 2 * w];
+    __syncthreads();
+  }
+  if (tid < 2) out[tid] = sh[tid];
 }
 
 // ------------------------------------------------------- launch helpers
@@ -896,6 +989,7 @@ struct tfm_pcg {
   double *res2 = nullptr;
   int *done = nullptr, *alldone = nullptr;
   int pq_cap = 0;                                          // rows of part_pq
+  int res_cap = 0;                                         // rows (pairs) of part_res
   int *h_done = nullptr;                                   // pinned host mirrors
   double *h_res = nullptr;
   // optional event timing of the SpMV launches (kernel A) of the iterations
@@ -925,9 +1019,10 @@ static int pcg_alloc(tfm_pcg *h) {
   // 8-lane share of a block, at most n / 8 + 1 blocks
   h->pq_cap = (int)std::max<int64_t>(RBA, n / 8 + 1);
   TFM_CHECK_HIP(hipMalloc(&h->part_pq, sizeof(double) * (size_t)h->pq_cap * nrp));
-  TFM_CHECK_HIP(hipMalloc(&h->part_upd, sizeof(double) * RBV * 2 * nrp));
-  TFM_CHECK_HIP(hipMalloc(&h->part_aux, sizeof(double) * RBV * nrp));
-  TFM_CHECK_HIP(hipMalloc(&h->part_res, sizeof(double) * RBV * 2));
+  TFM_CHECK_HIP(hipMalloc(&h->part_upd, sizeof(double) * RBW * 2 * nrp));
+  TFM_CHECK_HIP(hipMalloc(&h->part_aux, sizeof(double) * RBW * nrp));
+  h->res_cap = (int)std::max<int64_t>(RBV, n / 8 + 1);  // >= n_nod / 8 node teams
+  TFM_CHECK_HIP(hipMalloc(&h->part_res, sizeof(double) * (size_t)h->res_cap * 2));
   TFM_CHECK_HIP(hipMalloc(&h->coef, sizeof(PcgCoef) * 2));
   TFM_CHECK_HIP(hipMalloc(&h->res2, sizeof(double) * 2));
   TFM_CHECK_HIP(hipMalloc(&h->done, sizeof(int) * nrp));
@@ -1008,7 +1103,7 @@ template <typename T> struct Ops {
 };
 
 int sum_residual(tfm_pcg *h, hipStream_t s, unsigned nblk, double *rr, double *bb) {
-  hipLaunchKernelGGL(k_sum2, dim3(1), dim3(BS), 0, s, (const double *)h->part_res, (int)nblk, h->res2);
+  hipLaunchKernelGGL(k_sum2, dim3(1), dim3(1024), 0, s, (const double *)h->part_res, (int)nblk, h->res2);
   TFM_CHECK_LAUNCH();
   TFM_CHECK_HIP(hipMemcpyAsync(h->h_res, h->res2, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
   TFM_CHECK_HIP(hipStreamSynchronize(s));
@@ -1032,7 +1127,10 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
               h->part_pq,  h->part_upd, h->part_aux, h->pq_cap,   h->coef,
               h->coef + 1};
   const unsigned g1 = (unsigned)((n + 255) / 256);
-  const unsigned gv = grid_for(n_nod * NRP, BS, RBV);
+  // vector kernels: one (node, rhs) per thread when a scalar kernel sums
+  // their partials, else at most RBV blocks for the prologue of A
+  static const int vcap = std::min(RBW, std::max(1, env_int("TFM_VEC_GRID", RBV)));
+  const unsigned gv = grid_for(n_nod * NRP, BS, ops.scal ? RBW : vcap);
   std::vector<double> htol(NRP, -1.0), bnorm(NRP, 0.0), rprev(NRP, 0.0), rnow(NRP, 0.0);
   std::vector<int> fin(NRP, 1), stall(NRP, 0);
   for (int r = 0; r < nrhs; ++r) iters[r] = 0;
@@ -1211,8 +1309,11 @@ int fem_residual(tfm_pcg *h, hipStream_t s, const tfm_system_t &sys, const doubl
   const unsigned g1 = (unsigned)((n + 255) / 256);
   const uint8_t *con = nullptr;
   if (sys.val64) {
-    int rc = launch_spmv(s, m, 0, sys.val64, x, h->f64, 1);
-    if (rc != TFM_OK) return rc;
+    const unsigned gr = grid_for(m.n_nod, BS / 32, h->res_cap);  // one node per team
+    TFM_ND_SWITCH(m.ndof, hipLaunchKernelGGL((k_residual_dd<ND, 32>), dim3(gr), dim3(BS), 0, s, m,
+                                             sys.val64, b, x, res, h->part_res));
+    TFM_CHECK_LAUNCH();
+    return sum_residual(h, s, gr, rr, bb);
   } else {
     TFM_REQUIRE(sys.mat != nullptr, "matrix-free residual needs the material");
     const int64_t wl = m.n_elem * m.npe * m.ndof;

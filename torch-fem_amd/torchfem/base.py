@@ -66,7 +66,9 @@ class _System:
                               scale=self.scale.data_ptr() if self.scale is not None else None)
         self.con = fem._con_mask()
         self.precision = precision
-        self.val64 = torch.empty(pat.nnz, dtype=torch.float64, device=dev) if precision == "fp64" else None
+        # mixed precision keeps the FP64 values too: its refinement residuals
+        # b - K x are then one FP64 SpMV instead of a matrix-free pass
+        self.val64 = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
         self.val32 = torch.empty(pat.nnz, dtype=torch.float32, device=dev) if precision == "mixed" else None
         nd = pat.ndof
         self.diag = torch.empty(pat.n_nod * nd * nd, dtype=torch.float64, device=dev)

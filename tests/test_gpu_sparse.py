@@ -131,3 +131,56 @@ def test_resolve_and_describe():
     assert resolve_method(9999, "cuda", None) == "spsolve"
     assert resolve_method(10000, "cuda", None) == "cg"
     assert "jacobi" in describe_method(20000, "cuda", None)
+
+
+def _nonlinear_problem(n=120, seed=0):
+    """R(u; p) = A u + p * u^3 - f with the 1-D Dirichlet Laplacian A (SPD)
+    and a positive cubic term, so the tangent A + 3 p diag(u^2) stays SPD."""
+    g = torch.Generator().manual_seed(seed)
+    A = (2.0 * torch.eye(n) - torch.diag(torch.ones(n - 1), 1) - torch.diag(torch.ones(n - 1), -1))
+    A = A.double().cuda()
+    f = (1.0 + torch.rand(n, generator=g, dtype=torch.float64)).cuda()
+    return A, f
+
+
+def _residual_fn(A, f, p):
+    def eval_residual(du, i, u_prev, grad_prev, flux_prev, state_prev):
+        R = A @ du + p * du**3 - f
+        K = A + torch.diag(3.0 * p.detach() * du.detach() ** 2)
+        return R, K.to_sparse_coo().coalesce()
+    return eval_residual
+
+
+def test_newton_adjoint_gradient_matches_implicit_function_and_differences():
+    """sparse.py:517-795: Newton to rtol 1e-12, then d(sum u)/dp through the
+    adjoint against -1^T K^-1 u^3 (implicit function) and central differences."""
+    A, f = _nonlinear_problem()
+    p = torch.tensor(0.5, dtype=torch.float64, device="cuda", requires_grad=True)
+    u0 = torch.zeros(A.shape[0], dtype=torch.float64, device="cuda")
+    u = newton_solve(_residual_fn(A, f, p), u0, None, 30, 1e-12, 1e-14, 1e-12, None,
+                     None, None, None, False, None, None, None, None, p)
+    R = A @ u.detach() + 0.5 * u.detach() ** 3 - f
+    assert float(R.norm()) < 1e-10 * float(f.norm())
+    (g,) = torch.autograd.grad(u.sum(), p)
+    K = A + torch.diag(1.5 * u.detach() ** 2)
+    implicit = -torch.linalg.solve(K, u.detach() ** 3).sum()
+    assert abs(float(g) - float(implicit)) < 1e-9 * abs(float(implicit))
+    h = 1e-6
+    vals = []
+    for s in (1.0, -1.0):
+        ps = torch.tensor(0.5 + s * h, dtype=torch.float64, device="cuda")
+        us = newton_solve(_residual_fn(A, f, ps), u0, None, 30, 1e-13, 1e-14, 1e-12, None,
+                          None, None, None, False, None, None, None, None, ps)
+        vals.append(float(us.sum()))
+    fd = (vals[0] - vals[1]) / (2 * h)
+    assert abs(fd - float(g)) < 1e-6 * abs(float(g))
+
+
+def test_newton_raises_without_convergence():
+    """sparse.py:622-623: RuntimeError when max_iter runs out."""
+    A, f = _nonlinear_problem()
+    p = torch.tensor(50.0, dtype=torch.float64, device="cuda")
+    u0 = torch.zeros(A.shape[0], dtype=torch.float64, device="cuda")
+    with pytest.raises(RuntimeError, match="did not converge"):
+        newton_solve(_residual_fn(A, f, p), u0, None, 2, 1e-12, 1e-14, 1e-12, None,
+                     None, None, None, False, None, None, None, None, p)

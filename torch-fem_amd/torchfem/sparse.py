@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import time
+import weakref
 from collections.abc import Callable
 from typing import Any
 
@@ -90,26 +91,29 @@ class PCGSolver:
         self.nrhs = int(nrhs)
         self.max_iter = int(max_iter or max(100, min(10 * self.n, 20000)))
         self._h = C.c_void_p()
+        self._release = None
+        self._create()
+
+    def _create(self) -> None:
         with torch.cuda.device(self.device):
             L.check(L.lib().tfm_pcg_create(C.byref(self._h), self.n, self.nrhs, self.max_iter))
+        # no __del__ (it may run while the interpreter tears the library down,
+        # the reason amgx.py:13-20 gives); a finalizer releases an unclosed
+        # workspace at collection or at exit, while the modules are alive
+        self._release = weakref.finalize(self, L.lib().tfm_pcg_destroy, C.c_void_p(self._h.value))
 
     def close(self) -> None:
-        if self._h:
-            L.lib().tfm_pcg_destroy(self._h)
-            self._h = C.c_void_p()
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        """Releases the workspace (idempotent)."""
+        if self._release is not None:
+            self._release()
+            self._release = None
+        self._h = C.c_void_p()
 
     def _ensure(self, nrhs: int) -> None:
         if nrhs > self.nrhs:
             self.close()
             self.nrhs = nrhs
-            with torch.cuda.device(self.device):
-                L.check(L.lib().tfm_pcg_create(C.byref(self._h), self.n, self.nrhs, self.max_iter))
+            self._create()
 
     def timing(self, enable: bool) -> tuple[float, int]:
         """Event-timed inner SpMV launches since the last call (ms, count);

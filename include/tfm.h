@@ -100,6 +100,9 @@ typedef struct tfm_mesh {
   const int32_t *ne_elem;    /* [n_elem*npe] incident elements, ascending */
   const int32_t *ne_loc;     /* [n_elem*npe] local index of the node there */
   const uint16_t *epos;      /* [n_elem, npe, npe] position of node j in adj(node i) */
+  const int16_t *adj16;      /* optional [adj_ptr[n_nod]] adj[k] - (row node), when every
+                                such difference fits in 16 bits (NULL otherwise); the
+                                SpMV kernels then read half the index bytes */
 } tfm_mesh_t;
 
 /* Per-element material parameters (device). */

@@ -30,6 +30,7 @@
 namespace tfm {
 
 constexpr int BS = 256;   // threads per block of the solver kernels
+constexpr int BSV = 256;  // threads per block of the PCG vector update (kernel B)
 constexpr int RBA = 2048; // max blocks of the SpMV kernels (= eta partials per rhs)
 constexpr int RBV = 512;  // max blocks of the vector kernels whose partials every block of A sums
 constexpr int RBW = 4096; // max blocks of the vector kernels when a scalar kernel sums them
@@ -42,10 +43,10 @@ enum { S_RZ = 0, S_PQ = 1, S_RR = 2 };
 // block's sums go to part[blockIdx.x][k][r]. Each wave first folds its lanes
 // of equal t % NRP with a butterfly, then one LDS stage adds the waves (a
 // fixed order, so the result is reproducible).
-template <int NV, int NRP>
+template <int NV, int NRP, int BSZ = BS>
 __device__ __forceinline__ void write_block_partials(const double (&v)[NV], double *sh,
                                                      double *part) {
-  constexpr int NW = BS / 64;
+  constexpr int NW = BSZ / 64;
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   double w[NV];
 #pragma unroll
@@ -69,10 +70,10 @@ __device__ __forceinline__ void write_block_partials(const double (&v)[NV], doub
 
 // Sums the NVT partial slots written by nblk blocks (in block order) into
 // out[0..NVT) in LDS; identical in every block that calls it.
-template <int NVT>
+template <int NVT, int BSZ = BS>
 __device__ __forceinline__ void sum_partials(const double *part, int nblk, double *sh,
                                              double *out) {
-  constexpr int CH = BS / NVT;
+  constexpr int CH = BSZ / NVT;
   const int tid = threadIdx.x;
   double acc = 0.0;
   if (tid < CH * NVT) {
@@ -120,10 +121,12 @@ template <int TL, int NRP> constexpr int hoisted_blocks() { return (TL < 32 && N
 // blocks of every lane are loaded before any arithmetic, so a lane keeps
 // KB*(ND+1) loads in flight. The butterfly leaves the team sums in every
 // lane; output element e = a*NRP + r is handled by lane e % TL.
-template <typename T, int ND, int TL, int NRP>
+template <typename T, int ND, int TL, int NRP, bool D16>
 __device__ __forceinline__ void node_product(const tfm_mesh_t &m, const T *__restrict__ val,
                                              const T *__restrict__ x, int64_t n, int lane,
                                              T (&acc)[ND][NRP]) {
+  // neighbour k of the node: from the 16-bit offsets (D16) or the int32 indices
+  auto nbr = [&](int k) -> int64_t { return D16 ? n + (int64_t)m.adj16[k] : (int64_t)m.adj[k]; };
   constexpr int KB = hoisted_blocks<TL, NRP>();
 #pragma unroll
   for (int a = 0; a < ND; ++a)
@@ -149,7 +152,7 @@ __device__ __forceinline__ void node_product(const tfm_mesh_t &m, const T *__res
       const int p = lane + k * TL;
       col[k] = -1;
       if (p < deg) {
-        col[k] = m.adj[a0 + p];
+        col[k] = (int)nbr(a0 + p);
 #pragma unroll
         for (int a = 0; a < ND; ++a) rows[k][a] = ldv<T, ND>(vn + a * L + p * ND);
       }
@@ -161,7 +164,7 @@ __device__ __forceinline__ void node_product(const tfm_mesh_t &m, const T *__res
       Vec<T, ND> rw[ND];
 #pragma unroll
       for (int a = 0; a < ND; ++a) rw[a] = ldv<T, ND>(vn + a * L + p * ND);
-      fma_block(rw, ldv<T, ND * NRP>(x + (int64_t)m.adj[a0 + p] * ND * NRP));
+      fma_block(rw, ldv<T, ND * NRP>(x + nbr(a0 + p) * ND * NRP));
     }
   } else if (lane < ND) {  // orphan node: only its diagonal is stored
 #pragma unroll
@@ -179,14 +182,14 @@ __device__ __forceinline__ void node_product(const tfm_mesh_t &m, const T *__res
 }
 
 // y = K x over the node blocks (every right-hand side of the interleaved x).
-template <typename T, int ND, int TL, int NRP>
+template <typename T, int ND, int TL, int NRP, bool D16>
 __global__ void __launch_bounds__(BS) k_spmv(tfm_mesh_t m, const T *__restrict__ val,
                                              const T *__restrict__ x, T *__restrict__ y) {
   constexpr int TPB = BS / TL;
   const int lane = threadIdx.x % TL, team = threadIdx.x / TL;
   for (int64_t n = (int64_t)blockIdx.x * TPB + team; n < m.n_nod; n += (int64_t)gridDim.x * TPB) {
     T acc[ND][NRP];
-    node_product<T, ND, TL, NRP>(m, val, x, n, lane, acc);
+    node_product<T, ND, TL, NRP, D16>(m, val, x, n, lane, acc);
 #pragma unroll
     for (int a = 0; a < ND; ++a)
 #pragma unroll
@@ -353,7 +356,7 @@ __device__ __forceinline__ int dir_prologue(int it, int seg_first, const double 
 // element e = a*NRP + r (lane e % TL) belongs to rhs tid % NRP, the slot of
 // its eta partial. CO: beta and the running flags come from coef_a (prepared
 // by k_pcg_scal_dir) instead of the partial sets at part_in.
-template <typename T, int ND, int TL, int NRP, bool CO>
+template <typename T, int ND, int TL, int NRP, bool CO, bool D16>
 __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__restrict__ val,
                                               const T *__restrict__ u, T *__restrict__ p,
                                               T *__restrict__ s, int it, int seg_first,
@@ -396,7 +399,7 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
       }
     }
     T acc[ND][NRP];
-    node_product<T, ND, TL, NRP>(m, val, u, n, lane, acc);
+    node_product<T, ND, TL, NRP, D16>(m, val, u, n, lane, acc);
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
       const int e = lane + k * TL;
@@ -419,23 +422,23 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
   write_block_partials<1, NRP>(v, sh, st.part_pq);
 }
 
-template <typename T, int ND, int TL, int NRP, bool CO>
+template <typename T, int ND, int TL, int NRP, bool CO, bool D16>
 __global__ void __launch_bounds__(BS) k_pcg_spmv_dir(tfm_mesh_t m, const T *__restrict__ val,
                                                      const T *__restrict__ u, T *__restrict__ p,
                                                      T *__restrict__ s, int it, int seg_first,
                                                      const double *part_in, int n_in,
                                                      PcgState st) {
-  spmv_dir_body<T, ND, TL, NRP, CO>(m, val, u, p, s, it, seg_first, part_in, n_in, st);
+  spmv_dir_body<T, ND, TL, NRP, CO, D16>(m, val, u, p, s, it, seg_first, part_in, n_in, st);
 }
 
 // The same kernel held to 64 VGPRs (8 waves per SIMD), so that a grid of
 // 2048 blocks of 4 waves is resident in one round on 256 CUs.
-template <typename T, int ND, int TL, int NRP, bool CO>
+template <typename T, int ND, int TL, int NRP, bool CO, bool D16>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_pcg_spmv_dir_w8(tfm_mesh_t m, const T *__restrict__ val, const T *__restrict__ u,
                   T *__restrict__ p, T *__restrict__ s, int it, int seg_first,
                   const double *part_in, int n_in, PcgState st) {
-  spmv_dir_body<T, ND, TL, NRP, CO>(m, val, u, p, s, it, seg_first, part_in, n_in, st);
+  spmv_dir_body<T, ND, TL, NRP, CO, D16>(m, val, u, p, s, it, seg_first, part_in, n_in, st);
 }
 
 // A over a general CSR matrix (one rhs).
@@ -470,13 +473,13 @@ __global__ void __launch_bounds__(BS) k_pcg_csr_dir(int64_t n, const int64_t *ro
 // B: alpha = gamma / eta; x += alpha p; r -= alpha s; u = M r; partials of
 // (r.u, r.r) for the next A. pq_in holds n_pq partial sets of eta.
 template <typename T, int ND, int NRP, bool CO>
-__global__ void __launch_bounds__(BS) k_pcg_update(int64_t n_nod, int it, T *__restrict__ x,
+__global__ void __launch_bounds__(BSV) k_pcg_update(int64_t n_nod, int it, T *__restrict__ x,
                                                    T *__restrict__ r, T *__restrict__ u,
                                                    const T *__restrict__ p,
                                                    const T *__restrict__ s,
                                                    const T *__restrict__ minv, PcgState st,
                                                    const double *pq_in, int n_pq) {
-  __shared__ double sh[2 * BS];
+  __shared__ double sh[2 * BSV];
   __shared__ double pq[NRP];
   __shared__ double alpha[NRP];
   const int tid = threadIdx.x;
@@ -486,7 +489,7 @@ __global__ void __launch_bounds__(BS) k_pcg_update(int64_t n_nod, int it, T *__r
     if (tid < NRP) alpha[tid] = cf.c[tid];
   } else {
     if (*st.alldone) return;
-    sum_partials<NRP>(pq_in, n_pq, sh, pq);
+    sum_partials<NRP, BSV>(pq_in, n_pq, sh, pq);
     if (tid < NRP) {
       double *sc = st.scal + ((size_t)it * NRP + tid) * NSCAL;
       alpha[tid] = (st.done[tid] < 0 && pq[tid] != 0.0) ? sc[S_RZ] / pq[tid] : 0.0;
@@ -496,7 +499,7 @@ __global__ void __launch_bounds__(BS) k_pcg_update(int64_t n_nod, int it, T *__r
   __syncthreads();
   double acc[2] = {0.0, 0.0};
   const int64_t total = n_nod * NRP;
-  for (int64_t t = (int64_t)blockIdx.x * BS + tid; t < total; t += (int64_t)gridDim.x * BS) {
+  for (int64_t t = (int64_t)blockIdx.x * BSV + tid; t < total; t += (int64_t)gridDim.x * BSV) {
     const int64_t n = t / NRP;
     const int rr = (int)(t % NRP);
     const T al = (T)alpha[rr];
@@ -541,7 +544,7 @@ __global__ void __launch_bounds__(BS) k_pcg_update(int64_t n_nod, int it, T *__r
       for (int a = 0; a < ND; ++a) u[(n * ND + a) * NRP + rr] = zv[a];
     }
   }
-  write_block_partials<2, NRP>(acc, sh, st.part_upd);
+  write_block_partials<2, NRP, BSV>(acc, sh, st.part_upd);
 }
 
 // Scalar kernels (one block) for wide grids. k_pcg_scal_dir prepares A of
@@ -829,6 +832,18 @@ static unsigned grid_for(int64_t work, int per_block, int cap) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, cap));
 }
 
+// g rounded up to a multiple of the number of CUs of the current device.
+static unsigned round_to_cus(unsigned g) {
+  static const int cus = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 1;
+    return std::max(1, n);
+  }();
+  return (unsigned)(((int64_t)g + cus - 1) / cus * cus);
+}
+
 // Whether to launch the 64-VGPR variant of a kernel: when it needs no scratch
 // (TFM_SPMV_W8=0/1 in the environment forces the choice).
 static bool pick_w8(const void *kernel_w8) {
@@ -914,8 +929,37 @@ static int spmv_launch(hipStream_t s, const tfm_mesh_t &m, const T *val, const T
   const SpmvShape sh = spmv_shape<T>(m, NRP);
   TFM_TEAM_SWITCH(sh.tl, {
     const unsigned g = grid_for(m.n_nod, BS / TL, sh.cap);
-    hipLaunchKernelGGL((k_spmv<T, ND, TL, NRP>), dim3(g), dim3(BS), 0, s, m, val, x, y);
+    if (m.adj16)
+      hipLaunchKernelGGL((k_spmv<T, ND, TL, NRP, true>), dim3(g), dim3(BS), 0, s, m, val, x, y);
+    else
+      hipLaunchKernelGGL((k_spmv<T, ND, TL, NRP, false>), dim3(g), dim3(BS), 0, s, m, val, x, y);
   });
+  TFM_CHECK_LAUNCH();
+  return TFM_OK;
+}
+
+template <typename T, int ND, int TL, int NRP, bool D16>
+static int spmv_dir_launch_d(hipStream_t s, const SpmvShape &sh, const tfm_mesh_t &m,
+                             const T *val, const T *u, T *p, T *sv, int it, int seg_first,
+                             const double *part_in, int n_in, const PcgState &st, unsigned *g) {
+  if (sh.scal) {
+    *g = grid_for(m.n_nod, BS / TL, sh.cap);
+    TFM_REQUIRE((int64_t)*g <= st.pq_cap, "SpMV grid exceeds the partial-sum buffer");
+    hipLaunchKernelGGL((k_pcg_spmv_dir<T, ND, TL, NRP, true, D16>), dim3(*g), dim3(BS), 0, s, m,
+                       val, u, p, sv, it, seg_first, part_in, n_in, st);
+  } else {
+    // grid-stride over the nodes with every block resident in one round;
+    // the 8-wave variant unless holding it to 64 VGPRs spills registers
+    static const auto kern =
+        pick_w8((const void *)k_pcg_spmv_dir_w8<T, ND, TL, NRP, false, D16>)
+            ? k_pcg_spmv_dir_w8<T, ND, TL, NRP, false, D16>
+            : k_pcg_spmv_dir<T, ND, TL, NRP, false, D16>;
+    static const int resident = resident_grid((const void *)kern);
+    *g = grid_for(m.n_nod, BS / TL, std::min(sh.cap, resident));
+    TFM_REQUIRE((int64_t)*g <= st.pq_cap, "SpMV grid exceeds the partial-sum buffer");
+    hipLaunchKernelGGL(kern, dim3(*g), dim3(BS), 0, s, m, val, u, p, sv, it, seg_first, part_in,
+                       n_in, st);
+  }
   TFM_CHECK_LAUNCH();
   return TFM_OK;
 }
@@ -925,29 +969,15 @@ static int spmv_dir_launch(hipStream_t s, const tfm_mesh_t &m, const T *val, con
                            T *sv, int it, int seg_first, const double *part_in, int n_in,
                            const PcgState &st, unsigned *grid_out) {
   const SpmvShape sh = spmv_shape<T>(m, NRP);
-  unsigned g = 1;
+  int rc = TFM_OK;
   TFM_TEAM_SWITCH(sh.tl, {
-    if (sh.scal) {
-      g = grid_for(m.n_nod, BS / TL, sh.cap);
-      TFM_REQUIRE((int64_t)g <= st.pq_cap, "SpMV grid exceeds the partial-sum buffer");
-      hipLaunchKernelGGL((k_pcg_spmv_dir<T, ND, TL, NRP, true>), dim3(g), dim3(BS), 0, s, m, val,
-                         u, p, sv, it, seg_first, part_in, n_in, st);
-    } else {
-      // grid-stride over the nodes with every block resident in one round;
-      // the 8-wave variant unless holding it to 64 VGPRs spills registers
-      static const auto kern = pick_w8((const void *)k_pcg_spmv_dir_w8<T, ND, TL, NRP, false>)
-                                   ? k_pcg_spmv_dir_w8<T, ND, TL, NRP, false>
-                                   : k_pcg_spmv_dir<T, ND, TL, NRP, false>;
-      static const int resident = resident_grid((const void *)kern);
-      g = grid_for(m.n_nod, BS / TL, std::min(sh.cap, resident));
-      TFM_REQUIRE((int64_t)g <= st.pq_cap, "SpMV grid exceeds the partial-sum buffer");
-      hipLaunchKernelGGL(kern, dim3(g), dim3(BS), 0, s, m, val, u, p, sv, it, seg_first, part_in,
-                         n_in, st);
-    }
+    rc = m.adj16 ? spmv_dir_launch_d<T, ND, TL, NRP, true>(s, sh, m, val, u, p, sv, it, seg_first,
+                                                           part_in, n_in, st, grid_out)
+                 : spmv_dir_launch_d<T, ND, TL, NRP, false>(s, sh, m, val, u, p, sv, it,
+                                                            seg_first, part_in, n_in, st,
+                                                            grid_out);
   });
-  TFM_CHECK_LAUNCH();
-  *grid_out = g;
-  return TFM_OK;
+  return rc;
 }
 
 int launch_spmv(hipStream_t s, const tfm_mesh_t &m, int prec, const void *vals, const void *x,
@@ -1130,7 +1160,9 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
   // vector kernels: one (node, rhs) per thread when a scalar kernel sums
   // their partials, else at most RBV blocks for the prologue of A
   static const int vcap = std::min(RBW, std::max(1, env_int("TFM_VEC_GRID", RBV)));
-  const unsigned gv = grid_for(n_nod * NRP, BS, ops.scal ? RBW : vcap);
+  // whole multiples of the CU count, so that no CU runs a second round alone
+  const unsigned gv = std::min<unsigned>(ops.scal ? RBW : vcap,
+                                         round_to_cus(grid_for(n_nod * NRP, BSV, RBW)));
   std::vector<double> htol(NRP, -1.0), bnorm(NRP, 0.0), rprev(NRP, 0.0), rnow(NRP, 0.0);
   std::vector<int> fin(NRP, 1), stall(NRP, 0);
   for (int r = 0; r < nrhs; ++r) iters[r] = 0;
@@ -1215,11 +1247,11 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
           hipLaunchKernelGGL((k_pcg_scal_alpha<NRP>), dim3(1), dim3(1024), 0, s, it,
                              (const double *)h->part_pq, (int)npq, st);
           TFM_CHECK_LAUNCH();
-          hipLaunchKernelGGL((k_pcg_update<T, ND, NRP, true>), dim3(gv), dim3(BS), 0, s, n_nod, it,
+          hipLaunchKernelGGL((k_pcg_update<T, ND, NRP, true>), dim3(gv), dim3(BSV), 0, s, n_nod, it,
                              xv, rv, uv, (const T *)pv, (const T *)sv, minv, st,
                              (const double *)h->part_pq, (int)npq);
         } else {
-          hipLaunchKernelGGL((k_pcg_update<T, ND, NRP, false>), dim3(gv), dim3(BS), 0, s, n_nod, it,
+          hipLaunchKernelGGL((k_pcg_update<T, ND, NRP, false>), dim3(gv), dim3(BSV), 0, s, n_nod, it,
                              xv, rv, uv, (const T *)pv, (const T *)sv, minv, st,
                              (const double *)h->part_pq, (int)npq);
         }

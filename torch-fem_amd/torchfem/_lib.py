@@ -58,6 +58,7 @@ class Mesh(C.Structure):
         ("ne_elem", P),
         ("ne_loc", P),
         ("epos", P),
+        ("adj16", P),
     ]
 
 

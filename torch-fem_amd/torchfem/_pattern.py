@@ -79,6 +79,11 @@ class DevicePattern:
         self.ne_loc = up(host["ne_loc"])
         # uint16 travels as int16 storage (same bits)
         self.epos = torch.from_numpy(host["epos"].view(np.int16)).to(device)
+        # 16-bit neighbour offsets (adj[k] - row node) when they all fit
+        rows = np.repeat(np.arange(self.n_nod, dtype=np.int64), np.diff(host["adj_ptr"]))
+        delta = host["adj"].astype(np.int64) - rows
+        fits = delta.size > 0 and int(np.abs(delta).max()) <= np.iinfo(np.int16).max
+        self.adj16 = up(delta.astype(np.int16)) if fits else None
         self.mesh = L.Mesh(
             nodes=self.nodes.data_ptr(),
             elements=self.elements.data_ptr(),
@@ -98,6 +103,7 @@ class DevicePattern:
             ne_elem=self.ne_elem.data_ptr(),
             ne_loc=self.ne_loc.data_ptr(),
             epos=self.epos.data_ptr(),
+            adj16=self.adj16.data_ptr() if self.adj16 is not None else None,
         )
 
     @property

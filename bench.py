@@ -47,6 +47,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--fp64-steps", type=int, default=1, help="0 skips the FP64 timing")
     ap.add_argument("--cpu-n", type=int, default=32, help="cube size of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timing-stride", type=int, default=8,
+                    help="HIP-event timing of the SpMV kernel on every k-th PCG iteration "
+                         "(each timed launch adds an event pair to the stream; 0 disables)")
     ap.add_argument("--no-reference", action="store_true",
                     help="skip the tight FP64 solve used for the accuracy report")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -197,9 +200,9 @@ def main() -> None:
         torch.cuda.synchronize()
         log(f"mixed: warmup {i + 1}/{args.warmup} done")
     solver = model._solver()
-    solver.timing(True)
+    solver.timing(args.timing_stride)
     ms_mixed, (u_mix, gE_mix, gF_mix) = timed(step_mixed, args.steps, 0, "mixed")
-    spmv_ms, spmv_n = solver.timing(False)
+    spmv_ms, spmv_n = solver.timing(0)
     info_mixed = dict(model.last_solve_info)
 
     ms_fp64, info_fp64, u64, gE64 = None, None, None, None
@@ -231,11 +234,12 @@ def main() -> None:
     # roofline of the dominant kernel: the FP32 node-block SpMV fused with the
     # direction update (reads K, the adjacency, u once, p and s; writes p and s)
     n_adj = int(pat.adj.numel())
-    bytes_launch = pat.nnz * 4 + n_adj * 4 + (pat.n_nod + 1) * (4 + 8) + model.n_dofs * 4 * 5
+    idx_bytes = 2 if pat.adj16 is not None else 4  # 16-bit neighbour offsets when they fit
+    bytes_launch = pat.nnz * 4 + n_adj * idx_bytes + (pat.n_nod + 1) * (4 + 8) + model.n_dofs * 4 * 5
     avg_s = spmv_ms / max(spmv_n, 1) / 1e3
     achieved = bytes_launch / avg_s / 1e9 if spmv_n else None
     roof = {
-        "kernel": "k_pcg_spmv_dir<float,3,16,1,false>",
+        "kernel": "k_pcg_spmv_dir<float,3,16,1,false,true>",
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
@@ -244,6 +248,8 @@ def main() -> None:
         "traffic": measured_traffic(),
         "avg_launch_us": avg_s * 1e6,
         "launches": spmv_n,
+        "timing": f"HIP events on the solver stream around every {args.timing_stride}th "
+                  "iteration's SpMV launch during the timed steps",
         "algorithmic_bytes": bytes_launch,
     }
     it_fwd = sum(p["iterations"][0] for p in info_mixed["forward"]["pcg"])

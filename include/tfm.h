@@ -244,7 +244,8 @@ int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t s, const tfm_system_t *sys,
 
 /* Event timing of the inner SpMV launches of tfm_pcg_solve on the solve's
  * stream: returns the accumulated milliseconds and launch count since the
- * previous call, resets them, and enables (enable != 0) or disables timing.
+ * previous call, resets them, and sets the sampling: enable = k > 0 times the
+ * SpMV of every k-th iteration (an event pair per timed launch), 0 disables.
  * Measurement plumbing with no counterpart in the reference. */
 int tfm_pcg_timing(tfm_pcg_t *h, int32_t enable, double *spmv_ms,
                    int64_t *spmv_launches);

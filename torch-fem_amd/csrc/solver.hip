@@ -1022,11 +1022,12 @@ struct tfm_pcg {
   int res_cap = 0;                                         // rows (pairs) of part_res
   int *h_done = nullptr;                                   // pinned host mirrors
   double *h_res = nullptr;
+  double *h_scal = nullptr;                                // one iteration's scalars
   // optional event timing of the SpMV launches (kernel A) of the iterations
-  bool timing = false;
+  int timing = 0;  // 0: off; k: event pair around the SpMV of every k-th iteration
   double spmv_ms = 0.0;
   int64_t spmv_count = 0;
-  hipEvent_t ev[64] = {};
+  hipEvent_t ev[2 * 64] = {};                              // two per iteration of a batch
 };
 
 static int pow2_cap(int v) {
@@ -1059,6 +1060,7 @@ static int pcg_alloc(tfm_pcg *h) {
   TFM_CHECK_HIP(hipMalloc(&h->alldone, sizeof(int)));
   TFM_CHECK_HIP(hipHostMalloc(&h->h_done, sizeof(int) * (nrp + 1), hipHostMallocDefault));
   TFM_CHECK_HIP(hipHostMalloc(&h->h_res, sizeof(double) * 4, hipHostMallocDefault));
+  TFM_CHECK_HIP(hipHostMalloc(&h->h_scal, sizeof(double) * nrp * NSCAL, hipHostMallocDefault));
   for (auto &e : h->ev) TFM_CHECK_HIP(hipEventCreate(&e));
   return TFM_OK;
 }
@@ -1073,6 +1075,7 @@ static void pcg_free(tfm_pcg *h) {
   if (h->alldone) (void)hipFree(h->alldone);
   if (h->h_done) (void)hipHostFree(h->h_done);
   if (h->h_res) (void)hipHostFree(h->h_res);
+  if (h->h_scal) (void)hipHostFree(h->h_scal);
   for (auto &e : h->ev)
     if (e) (void)hipEventDestroy(e);
 }
@@ -1112,7 +1115,7 @@ extern "C" int tfm_pcg_timing(tfm_pcg_t *h, int32_t enable, double *spmv_ms,
   if (spmv_launches) *spmv_launches = h->spmv_count;
   h->spmv_ms = 0.0;
   h->spmv_count = 0;
-  h->timing = enable != 0;
+  h->timing = enable > 0 ? enable : 0;
   return TFM_OK;
 }
 
@@ -1223,26 +1226,32 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
     if (rc != TFM_OK) return rc;
   }
 
-  const int CHK = 16;
+  // Batches between host checks: the first of a segment has CHK iterations;
+  // later ones are sized from the contraction of rr observed so far, to end
+  // the segment with few empty launches and few host round trips.
+  const int CHK = 16, CHK_MIN = 4, CHK_MAX = 64;
   int it = 0, n_outer = 0, seg_first = 0;
   bool any_live = mask != 0;
   while (any_live) {
-    // run the current segment in batches of CHK iterations
     bool seg_done = false;
+    int chk = CHK;
+    std::vector<double> rr_seen(NRP, -1.0);
+    std::vector<int> it_seen(NRP, -1);
     while (!seg_done && it < h->max_iter) {
       const int start = it;
-      const int stop = std::min(h->max_iter, it + CHK);
+      const int stop = std::min(h->max_iter, it + chk);
       for (; it < stop; ++it) {
         if (ops.scal) {
           hipLaunchKernelGGL((k_pcg_scal_dir<NRP>), dim3(1), dim3(BS), 0, s, it, seg_first,
                              (const double *)h->part_upd, (int)gv, st);
           TFM_CHECK_LAUNCH();
         }
-        if (h->timing) TFM_CHECK_HIP(hipEventRecord(h->ev[2 * (it - start)], s));
+        const bool timed = h->timing && it % h->timing == 0;
+        if (timed) TFM_CHECK_HIP(hipEventRecord(h->ev[2 * (it - start)], s));
         unsigned npq = 0;
         int rc = ops.spmv_dir(uv, pv, sv, it, seg_first, h->part_upd, (int)gv, st, &npq);
         if (rc != TFM_OK) return rc;
-        if (h->timing) TFM_CHECK_HIP(hipEventRecord(h->ev[2 * (it - start) + 1], s));
+        if (timed) TFM_CHECK_HIP(hipEventRecord(h->ev[2 * (it - start) + 1], s));
         if (ops.scal) {
           hipLaunchKernelGGL((k_pcg_scal_alpha<NRP>), dim3(1), dim3(1024), 0, s, it,
                              (const double *)h->part_pq, (int)npq, st);
@@ -1258,6 +1267,8 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
         TFM_CHECK_LAUNCH();
       }
       TFM_CHECK_HIP(hipMemcpyAsync(h->h_done, h->done, sizeof(int) * NRP, hipMemcpyDeviceToHost, s));
+      TFM_CHECK_HIP(hipMemcpyAsync(h->h_scal, h->scal + (size_t)(stop - 1) * NRP * NSCAL,
+                                   sizeof(double) * NRP * NSCAL, hipMemcpyDeviceToHost, s));
       TFM_CHECK_HIP(hipStreamSynchronize(s));
       seg_done = true;
       int end = 0;
@@ -1265,9 +1276,26 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
         seg_done &= h->h_done[r] >= 0;
         end = std::max(end, h->h_done[r]);
       }
+      // next batch: iterations until the slowest running rhs reaches its
+      // segment target at its observed rate (rr of iteration stop-1)
+      int want = 0;
+      for (int r = 0; r < nrhs; ++r) {
+        if (h->h_done[r] >= 0) continue;
+        const double rr = h->h_scal[r * NSCAL + S_RR];
+        int k = CHK;
+        if (it_seen[r] >= 0 && rr > 0.0 && rr < rr_seen[r] && htol[r] > 0.0) {
+          const double rate = std::log(rr / rr_seen[r]) / (stop - 1 - it_seen[r]);
+          k = (int)std::ceil(std::log(htol[r] / rr) / rate) + 2;
+        }
+        rr_seen[r] = rr;
+        it_seen[r] = stop - 1;
+        want = std::max(want, std::min(std::max(k, CHK_MIN), CHK_MAX));
+      }
+      chk = want > 0 ? want : CHK;
       if (h->timing) {  // launches after the last rhs stopped were empty
         const int real_end = seg_done ? std::min(it, end) : it;
         for (int i = start; i < real_end; ++i) {
+          if (i % h->timing) continue;
           float ms = 0.0f;
           TFM_CHECK_HIP(hipEventElapsedTime(&ms, h->ev[2 * (i - start)], h->ev[2 * (i - start) + 1]));
           h->spmv_ms += ms;

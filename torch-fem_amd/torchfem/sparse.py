@@ -115,12 +115,13 @@ class PCGSolver:
             self.nrhs = nrhs
             self._create()
 
-    def timing(self, enable: bool) -> tuple[float, int]:
+    def timing(self, stride: int) -> tuple[float, int]:
         """Event-timed inner SpMV launches since the last call (ms, count);
-        resets the totals and switches timing on or off."""
+        resets the totals, then times the SpMV of every `stride`-th
+        iteration from now on (0 switches timing off)."""
         ms = L.F64()
         cnt = L.I64()
-        L.check(L.lib().tfm_pcg_timing(self._h, 1 if enable else 0, C.byref(ms), C.byref(cnt)))
+        L.check(L.lib().tfm_pcg_timing(self._h, int(stride), C.byref(ms), C.byref(cnt)))
         return float(ms.value), int(cnt.value)
 
     def solve_system(self, sys: L.System, b: Tensor, x: Tensor, precision="mixed",

@@ -1362,13 +1362,18 @@ int build_precond(hipStream_t s, int64_t n_nod, int nd, const double *dblk, int 
 
 // FP64 true residual of the finite-element system: with the FP64 values when
 // present, otherwise matrix-free through the element kernels.
+// extended: double-double row sums (tolerances near the FP64 floor of
+// eps |K| |x|, below 1e-10); otherwise one FP64 SpMV.
 int fem_residual(tfm_pcg *h, hipStream_t s, const tfm_system_t &sys, const double *b,
-                 const double *x, double *res, double *rr, double *bb) {
+                 const double *x, double *res, double *rr, double *bb, bool extended) {
   const tfm_mesh_t &m = *sys.mesh;
   const int64_t n = m.n_nod * m.ndof;
   const unsigned g1 = (unsigned)((n + 255) / 256);
   const uint8_t *con = nullptr;
-  if (sys.val64) {
+  if (sys.val64 && !extended) {
+    int rc = launch_spmv(s, m, 0, sys.val64, x, h->f64, 1);
+    if (rc != TFM_OK) return rc;
+  } else if (sys.val64) {
     const unsigned gr = grid_for(m.n_nod, BS / 32, h->res_cap);  // one node per team
     TFM_ND_SWITCH(m.ndof, hipLaunchKernelGGL((k_residual_dd<ND, 32>), dim3(gr), dim3(BS), 0, s, m,
                                              sys.val64, b, x, res, h->part_res));
@@ -1420,8 +1425,9 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
   hipStream_t s = (hipStream_t)stream;
   const int nrp = pow2_cap(nrhs);
   int rc = TFM_OK;
+  const bool extended = rtol < 1e-10;
   auto residual = [&](const double *bb, const double *xx, double *res, double *rr, double *b2) -> int {
-    return fem_residual(h, s, *sys, bb, xx, res, rr, b2);
+    return fem_residual(h, s, *sys, bb, xx, res, rr, b2, extended);
   };
   TFM_ND_SWITCH(m.ndof, TFM_NRP_SWITCH(nrp, {
     if (prec == TFM_PREC_FP64) {

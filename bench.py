@@ -105,7 +105,7 @@ def make_step(model, precision: str, stol: float, pc: str, inner_rtol: float):
         u, f, sigma, _, _ = model.solve(differentiable_parameters=(E, F), precision=precision,
                                         stol=stol, preconditioner=pc, inner_rtol=inner_rtol)
         gE, gF = torch.autograd.grad(u.sum(), (E, F))
-        return u.detach(), gE, gF
+        return u.detach(), gE, gF, f.detach(), sigma.detach()
 
     return step
 
@@ -224,14 +224,14 @@ def main() -> None:
         log(f"mixed: warmup {i + 1}/{args.warmup} done")
     solver = model._solver()
     solver.timing(args.timing_stride)
-    ms_mixed, (u_mix, gE_mix, gF_mix) = timed(step_mixed, args.steps, 0, "mixed")
+    ms_mixed, (u_mix, gE_mix, gF_mix, f_mix, s_mix) = timed(step_mixed, args.steps, 0, "mixed")
     spmv_ms, spmv_n = solver.timing(0)
     info_mixed = dict(model.last_solve_info)
 
     ms_fp64, info_fp64, u64, gE64 = None, None, None, None
     if args.fp64_steps > 0:
         step_fp64 = make_step(model, "fp64", args.stol, args.preconditioner, args.inner_rtol)
-        ms_fp64, (u64, gE64, _) = timed(step_fp64, args.fp64_steps, 1, "fp64")
+        ms_fp64, (u64, gE64, _, _, _) = timed(step_fp64, args.fp64_steps, 1, "fp64")
         info_fp64 = dict(model.last_solve_info)
 
     # FP64 reference of the same problem solved tightly, for the error report
@@ -241,10 +241,12 @@ def main() -> None:
     }
     if not args.no_reference:
         step_ref = make_step(model, "fp64", 1e-11, args.preconditioner, args.inner_rtol)
-        u_ref, gE_ref, gF_ref = step_ref()
+        u_ref, gE_ref, gF_ref, f_ref, s_ref = step_ref()
         rel = lambda a, b: float((a - b).norm() / b.norm())  # noqa: E731
         acc.update({
             "u_rel_err_vs_fp64": rel(u_mix, u_ref),
+            "f_rel_err_vs_fp64": rel(f_mix, f_ref),
+            "sigma_rel_err_vs_fp64": rel(s_mix, s_ref),
             "gradE_rel_err_vs_fp64": rel(gE_mix, gE_ref),
             "gradF_rel_err_vs_fp64": rel(gF_mix, gF_ref),
             "reference": "all-FP64 GPU PCG to 1e-11, whose parity with the CPU oracle is tested",

@@ -24,7 +24,6 @@ which is exact for a linear model.
 from __future__ import annotations
 
 import ctypes as C
-import math
 import time
 from abc import ABC, abstractmethod
 from collections.abc import Iterable

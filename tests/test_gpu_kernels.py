@@ -13,11 +13,11 @@ import scipy.sparse as sp
 import torch
 
 import oracle as O
-from torchfem import Planar, Solid, SolidHeat
+from torchfem import Planar, PlanarHeat, Solid, SolidHeat
 from torchfem import _lib as L
 from torchfem.base import _MatHolder, _System
-from torchfem.materials import (IsotropicConductivity3D, IsotropicElasticity3D,
-                                IsotropicElasticityPlaneStress)
+from torchfem.materials import (IsotropicConductivity2D, IsotropicConductivity3D,
+                                IsotropicElasticity3D, IsotropicElasticityPlaneStress)
 
 pytestmark = pytest.mark.gpu
 DT = torch.float64
@@ -31,14 +31,39 @@ def _case(name, general=False):
         E = 800.0 + 400.0 * torch.rand(len(el), generator=g, dtype=DT)
         t = 0.5 + torch.rand(len(el), generator=g, dtype=DT)
         mat = IsotropicElasticityPlaneStress(E, 0.3)
+        Cfull = O.plane_stress_C(E, torch.tensor(0.3))
+        if general:
+            Cfull = Cfull.clone()
+            Cfull[:, 0, 0, 0, 0] *= 1.5
+            Cfull[:, 0, 1, 0, 1] *= 0.7
+            Cfull[:, 1, 0, 1, 0] *= 0.7
+            Cfull[:, 0, 1, 1, 0] *= 0.7
+            Cfull[:, 1, 0, 0, 1] *= 0.7
+            mat = IsotropicElasticityPlaneStress(E, 0.3).vectorize(len(el))
+            mat.C = Cfull.clone()
         m = Planar(nodes.cuda(), el.cuda(), mat, thickness=t.cuda())
-        o = O.OracleModel(nodes, el, O.Quad1, "planar", O.plane_stress_C(E, torch.tensor(0.3)), thickness=t)
+        o = O.OracleModel(nodes, el, O.Quad1, "planar", Cfull, thickness=t)
+        return m, o
+    if name == "Quad1-heat":
+        nodes, el = O.rect_quad(6, 5, 2.0, 1.0)
+        k = torch.exp(0.5 * torch.randn(len(el), generator=g, dtype=DT))
+        t = 0.5 + torch.rand(len(el), generator=g, dtype=DT)
+        m = PlanarHeat(nodes.cuda(), el.cuda(), IsotropicConductivity2D(k), thickness=t.cuda())
+        o = O.OracleModel(nodes, el, O.Quad1, "heat", O.iso_KAPPA(k, dim=2), thickness=t)
         return m, o
     if name == "Tetra1-heat":
         nodes, el = O.cube_tetra6(3, jitter=0.1, seed=0)
         k = torch.exp(0.5 * torch.randn(len(el), generator=g, dtype=DT))
-        m = SolidHeat(nodes.cuda(), el.cuda(), IsotropicConductivity3D(k))
-        o = O.OracleModel(nodes, el, O.Tetra1, "heat", O.iso_KAPPA(k))
+        mat = IsotropicConductivity3D(k)
+        KAPPA = O.iso_KAPPA(k)
+        if general:
+            # an anisotropic SPD conductivity exercises the general layout
+            R = 0.3 * torch.randn(len(el), 3, 3, generator=g, dtype=DT)
+            KAPPA = KAPPA + R @ R.transpose(1, 2)
+            mat = IsotropicConductivity3D(k)
+            mat.KAPPA = KAPPA.clone()
+        m = SolidHeat(nodes.cuda(), el.cuda(), mat)
+        o = O.OracleModel(nodes, el, O.Tetra1, "heat", KAPPA)
         return m, o
     if name == "Tetra1":
         nodes, el = O.cube_tetra6(2, jitter=0.1, seed=3)
@@ -66,7 +91,8 @@ def _case(name, general=False):
     return m, o
 
 
-CASES = [("Quad1", False), ("Tetra1", False), ("Tetra1-heat", False), ("Hexa1", False),
+CASES = [("Quad1", False), ("Quad1", True), ("Quad1-heat", False), ("Tetra1", False),
+         ("Tetra1", True), ("Tetra1-heat", False), ("Tetra1-heat", True), ("Hexa1", False),
          ("Hexa1", True), ("Hexa2", False), ("Hexa2", True)]
 
 
@@ -202,5 +228,8 @@ def test_sensitivity_matches_autograd(name, general):
     elif kind == L.MAT_HEAT_ISO:
         ref0 = torch.einsum("eij,ij->e", gC, torch.eye(d, dtype=DT))
         assert torch.allclose(out[:, 0], ref0, rtol=1e-10, atol=1e-12 * ref0.abs().max())
+    elif kind == L.MAT_HEAT_GENERAL:
+        ref = gC.reshape(E, -1)
+        assert torch.allclose(out[:, : d**2], ref, rtol=1e-10, atol=1e-12 * ref.abs().max())
     if tk is not None:
         assert torch.allclose(out[:, -1], grads[1], rtol=1e-10, atol=1e-12 * grads[1].abs().max())

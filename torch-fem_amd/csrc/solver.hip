@@ -1180,6 +1180,11 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
       return TFM_ERR_NOCONV;
     }
     if (first) bnorm[r] = std::sqrt(bb2);
+    if (first && bb2 == 0.0) {  // b = 0: x = 0 whatever the initial guess
+      hipLaunchKernelGGL((k_fill_zero<double>), dim3(g1), dim3(256), 0, s, n, x + (int64_t)r * n);
+      TFM_CHECK_LAUNCH();
+      rr2 = 0.0;
+    }
     const double rn = std::sqrt(rr2);
     rprev[r] = first ? rn : rnow[r];
     rnow[r] = rn;

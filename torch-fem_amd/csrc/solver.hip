@@ -68,24 +68,37 @@ __device__ __forceinline__ void write_block_partials(const double (&v)[NV], doub
   }
 }
 
-// Sums the NVT partial slots written by nblk blocks (in block order) into
-// out[0..NVT) in LDS; identical in every block that calls it.
+// Sums the NVT partial slots written by nblk blocks into out[0..NVT) in LDS,
+// in a fixed order, so that every block that calls it gets the same bits.
+// Thread t adds the partials b = t/NVT + k*CH of slot t % NVT with four
+// loads in flight; a wave butterfly and one LDS stage over the waves follow
+// (a serial chain through LDS here used to cost microseconds per kernel).
 template <int NVT, int BSZ = BS>
 __device__ __forceinline__ void sum_partials(const double *part, int nblk, double *sh,
                                              double *out) {
-  constexpr int CH = BSZ / NVT;
-  const int tid = threadIdx.x;
-  double acc = 0.0;
-  if (tid < CH * NVT) {
-    const int slot = tid % NVT, c = tid / NVT;
-    for (int b = c; b < nblk; b += CH) acc += part[(size_t)b * NVT + slot];
+  static_assert(64 % NVT == 0 && BSZ % 64 == 0, "slots must tile a wave");
+  constexpr int CH = BSZ / NVT, NW = BSZ / 64;
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int slot = tid % NVT;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int b = tid / NVT;
+  for (; b + 3 * CH < nblk; b += 4 * CH) {
+    a0 += part[(size_t)b * NVT + slot];
+    a1 += part[(size_t)(b + CH) * NVT + slot];
+    a2 += part[(size_t)(b + 2 * CH) * NVT + slot];
+    a3 += part[(size_t)(b + 3 * CH) * NVT + slot];
   }
-  __syncthreads();
-  sh[tid] = acc;
+  for (; b < nblk; b += CH) a0 += part[(size_t)b * NVT + slot];
+  double acc = (a0 + a1) + (a2 + a3);
+#pragma unroll
+  for (int off = NVT; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
+  __syncthreads();  // sh may still be read by the caller
+  if (lane < NVT) sh[wave * NVT + lane] = acc;
   __syncthreads();
   if (tid < NVT) {
     double t = 0.0;
-    for (int c = 0; c < CH; ++c) t += sh[c * NVT + tid];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) t += sh[q * NVT + tid];
     out[tid] = t;
   }
   __syncthreads();

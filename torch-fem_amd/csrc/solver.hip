@@ -134,10 +134,22 @@ template <int TL, int NRP> constexpr int hoisted_blocks() { return (TL < 32 && N
 // blocks of every lane are loaded before any arithmetic, so a lane keeps
 // KB*(ND+1) loads in flight. The butterfly leaves the team sums in every
 // lane; output element e = a*NRP + r is handled by lane e % TL.
+// Where node n's row data start: its first adjacency slot, its degree and
+// its first value. Loaded ahead of the product by the PCG kernel.
+struct NodePtr {
+  int a0, deg;
+  int64_t v0;
+};
+
+__device__ __forceinline__ NodePtr node_ptr(const tfm_mesh_t &m, int64_t n) {
+  const int a0 = m.adj_ptr[n];
+  return NodePtr{a0, m.adj_ptr[n + 1] - a0, m.val_ptr[n]};
+}
+
 template <typename T, int ND, int TL, int NRP, bool D16>
-__device__ __forceinline__ void node_product(const tfm_mesh_t &m, const T *__restrict__ val,
-                                             const T *__restrict__ x, int64_t n, int lane,
-                                             T (&acc)[ND][NRP]) {
+__device__ __forceinline__ void node_product_at(const tfm_mesh_t &m, const T *__restrict__ val,
+                                                const T *__restrict__ x, int64_t n,
+                                                const NodePtr &np, int lane, T (&acc)[ND][NRP]) {
   // neighbour k of the node: from the 16-bit offsets (D16) or the int32 indices
   auto nbr = [&](int k) -> int64_t { return D16 ? n + (int64_t)m.adj16[k] : (int64_t)m.adj[k]; };
   constexpr int KB = hoisted_blocks<TL, NRP>();
@@ -145,9 +157,9 @@ __device__ __forceinline__ void node_product(const tfm_mesh_t &m, const T *__res
   for (int a = 0; a < ND; ++a)
 #pragma unroll
     for (int r = 0; r < NRP; ++r) acc[a][r] = T(0);
-  const int a0 = m.adj_ptr[n];
-  const int deg = m.adj_ptr[n + 1] - a0;
-  const T *vn = val + m.val_ptr[n];
+  const int a0 = np.a0;
+  const int deg = np.deg;
+  const T *vn = val + np.v0;
   auto fma_block = [&](const Vec<T, ND> (&rows)[ND], const Vec<T, ND * NRP> &xv) {
 #pragma unroll
     for (int a = 0; a < ND; ++a)
@@ -192,6 +204,13 @@ __device__ __forceinline__ void node_product(const tfm_mesh_t &m, const T *__res
     for (int a = 0; a < ND; ++a)
 #pragma unroll
       for (int r = 0; r < NRP; ++r) acc[a][r] += __shfl_xor(acc[a][r], off, TL);
+}
+
+template <typename T, int ND, int TL, int NRP, bool D16>
+__device__ __forceinline__ void node_product(const tfm_mesh_t &m, const T *__restrict__ val,
+                                             const T *__restrict__ x, int64_t n, int lane,
+                                             T (&acc)[ND][NRP]) {
+  node_product_at<T, ND, TL, NRP, D16>(m, val, x, n, node_ptr(m, n), lane, acc);
 }
 
 // y = K x over the node blocks (every right-hand side of the interleaved x).
@@ -332,12 +351,15 @@ struct SegInfo {
 
 // Prologue of A: the scalars of iteration `it`, identical in every block (the
 // partials are summed in block order); block 0 records them and the stop
-// flags. Returns nonzero while some rhs is running.
+// flags. Returns nonzero while some rhs is running. `ended` (the alldone flag
+// read at kernel entry, so that its load overlaps the partial loads) makes it
+// return 0 before recording anything.
 template <int NRP>
 __device__ __forceinline__ int dir_prologue(int it, int seg_first, const double *part_in, int n_in,
                                             const PcgState &st, double *sh, double *tot,
-                                            double *beta, int *live) {
+                                            double *beta, int *live, int ended = 0) {
   sum_partials<2 * NRP>(part_in, n_in, sh, tot);
+  if (ended) return 0;
   const int tid = threadIdx.x;
   if (tid < NRP) {
     const double g = tot[tid], rr = tot[NRP + tid];
@@ -379,6 +401,18 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
   __shared__ double sh[BS];
   __shared__ double tot[2 * NRP], beta[NRP];
   __shared__ int live[NRP];
+  constexpr int TPB = BS / TL;
+  const int lane = threadIdx.x % TL, team = threadIdx.x / TL;
+  const int64_t stride = (int64_t)gridDim.x * TPB;
+  int64_t n = (int64_t)blockIdx.x * TPB + team;
+  // grid-stride mode: row pointers run one node ahead of the product, so
+  // each node's loads start from pointers already in registers (the first
+  // ones load during the prologue). The full grid of the CO mode has one
+  // node per team and loads them in place.
+  NodePtr np{0, 0, 0};
+  if constexpr (!CO) {
+    if (n < m.n_nod) np = node_ptr(m, n);
+  }
   if constexpr (CO) {
     const PcgCoef &cf = *st.coef_a;
     if (!cf.any) return;
@@ -388,18 +422,23 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
     }
     __syncthreads();
   } else {
-    if (*st.alldone) return;
-    if (!dir_prologue<NRP>(it, seg_first, part_in, n_in, st, sh, tot, beta, live)) return;
+    const int ended = *st.alldone;
+    if (!dir_prologue<NRP>(it, seg_first, part_in, n_in, st, sh, tot, beta, live, ended)) return;
   }
-  constexpr int TPB = BS / TL;
   // lane l updates the elements e = l, l + TL, ... of the node's ND*NRP
   // (offset e from the node's first entry); they all belong to rhs l % NRP
   constexpr int NE = ND * NRP, KE = (NE + TL - 1) / TL;
-  const int lane = threadIdx.x % TL, team = threadIdx.x / TL;
   const T b = (T)beta[lane % NRP];
   const bool lv = live[lane % NRP] != 0;
   double dot = 0.0;
-  for (int64_t n = (int64_t)blockIdx.x * TPB + team; n < m.n_nod; n += (int64_t)gridDim.x * TPB) {
+  for (; n < m.n_nod; n += stride) {
+    NodePtr cur;
+    if constexpr (!CO) {
+      cur = np;
+      if (n + stride < m.n_nod) np = node_ptr(m, n + stride);
+    } else {
+      cur = node_ptr(m, n);
+    }
     const int64_t nb = n * NE;
     T up[KE], pp[KE], sp[KE];  // issued with the SpMV loads, used after it
 #pragma unroll
@@ -412,7 +451,7 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
       }
     }
     T acc[ND][NRP];
-    node_product<T, ND, TL, NRP, D16>(m, val, u, n, lane, acc);
+    node_product_at<T, ND, TL, NRP, D16>(m, val, u, n, cur, lane, acc);
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
       const int e = lane + k * TL;
@@ -501,8 +540,9 @@ __global__ void __launch_bounds__(BSV) k_pcg_update(int64_t n_nod, int it, T *__
     if (!cf.any) return;
     if (tid < NRP) alpha[tid] = cf.c[tid];
   } else {
-    if (*st.alldone) return;
+    const int ended = *st.alldone;  // tested after the sums, so the loads overlap
     sum_partials<NRP, BSV>(pq_in, n_pq, sh, pq);
+    if (ended) return;
     if (tid < NRP) {
       double *sc = st.scal + ((size_t)it * NRP + tid) * NSCAL;
       alpha[tid] = (st.done[tid] < 0 && pq[tid] != 0.0) ? sc[S_RZ] / pq[tid] : 0.0;

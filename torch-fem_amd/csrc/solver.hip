@@ -342,7 +342,28 @@ struct PcgState {
   int pq_cap;        // blocks of A whose partials part_pq can hold
   PcgCoef *coef_a;   // beta of A (scalar-kernel mode)
   PcgCoef *coef_b;   // alpha of B (scalar-kernel mode)
+  int xcd;           // grid-stride kernels give each XCD one contiguous share (block_span)
 };
+
+// Items [first, end) with step `stride` of a block of a grid-stride kernel,
+// per_block items per round. Workgroups go to the 8 XCDs round-robin (block
+// b on XCD b % 8); with `xcd` the blocks of one XCD share one contiguous
+// eighth of the items, so what an XCD gathers and streams stays in its L2.
+// The eighths are whole multiples of per_block, so item t is still handled
+// by thread t % per_block (the rhs slot of the partial sums).
+struct Span {
+  int64_t first, end, stride;
+};
+
+__device__ __forceinline__ Span block_span(int64_t total, int per_block, int xcd) {
+  if (xcd && gridDim.x % 8 == 0) {
+    const int64_t c = ((total + 7) / 8 + per_block - 1) / per_block * per_block;
+    const int64_t f = (int64_t)(blockIdx.x % 8) * c;
+    return Span{f + (int64_t)(blockIdx.x / 8) * per_block, f + c < total ? f + c : total,
+                (int64_t)(gridDim.x / 8) * per_block};
+  }
+  return Span{(int64_t)blockIdx.x * per_block, total, (int64_t)gridDim.x * per_block};
+}
 
 // Host view of where every rhs stopped, passed by value to the segment start.
 struct SegInfo {
@@ -403,15 +424,16 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
   __shared__ int live[NRP];
   constexpr int TPB = BS / TL;
   const int lane = threadIdx.x % TL, team = threadIdx.x / TL;
-  const int64_t stride = (int64_t)gridDim.x * TPB;
-  int64_t n = (int64_t)blockIdx.x * TPB + team;
+  const Span sp_ = block_span(m.n_nod, TPB, CO ? 0 : st.xcd);
+  const int64_t stride = sp_.stride, n_end = sp_.end;
+  int64_t n = sp_.first + team;
   // grid-stride mode: row pointers run one node ahead of the product, so
   // each node's loads start from pointers already in registers (the first
   // ones load during the prologue). The full grid of the CO mode has one
   // node per team and loads them in place.
   NodePtr np{0, 0, 0};
   if constexpr (!CO) {
-    if (n < m.n_nod) np = node_ptr(m, n);
+    if (n < n_end) np = node_ptr(m, n);
   }
   if constexpr (CO) {
     const PcgCoef &cf = *st.coef_a;
@@ -431,11 +453,11 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
   const T b = (T)beta[lane % NRP];
   const bool lv = live[lane % NRP] != 0;
   double dot = 0.0;
-  for (; n < m.n_nod; n += stride) {
+  for (; n < n_end; n += stride) {
     NodePtr cur;
     if constexpr (!CO) {
       cur = np;
-      if (n + stride < m.n_nod) np = node_ptr(m, n + stride);
+      if (n + stride < n_end) np = node_ptr(m, n + stride);
     } else {
       cur = node_ptr(m, n);
     }
@@ -551,8 +573,8 @@ __global__ void __launch_bounds__(BSV) k_pcg_update(int64_t n_nod, int it, T *__
   }
   __syncthreads();
   double acc[2] = {0.0, 0.0};
-  const int64_t total = n_nod * NRP;
-  for (int64_t t = (int64_t)blockIdx.x * BSV + tid; t < total; t += (int64_t)gridDim.x * BSV) {
+  const Span sp_ = block_span(n_nod * NRP, BSV, CO ? 0 : st.xcd);
+  for (int64_t t = sp_.first + tid; t < sp_.end; t += sp_.stride) {
     const int64_t n = t / NRP;
     const int rr = (int)(t % NRP);
     const T al = (T)alpha[rr];
@@ -1209,9 +1231,10 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
   const int64_t n = n_nod * ND;
   T *xv = (T *)h->x, *rv = (T *)h->r, *uv = (T *)h->u, *pv = (T *)h->p, *sv = (T *)h->s;
   const T *minv = (const T *)h->minv;
+  static const int xcd = env_int("TFM_XCD_MAP", 1);
   PcgState st{h->scal,     h->tol2,     h->rzold,    h->done,     h->alldone,
               h->part_pq,  h->part_upd, h->part_aux, h->pq_cap,   h->coef,
-              h->coef + 1};
+              h->coef + 1, xcd};
   const unsigned g1 = (unsigned)((n + 255) / 256);
   // vector kernels: one (node, rhs) per thread when a scalar kernel sums
   // their partials, else at most RBV blocks for the prologue of A

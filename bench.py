@@ -47,7 +47,7 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--fp64-steps", type=int, default=1, help="0 skips the FP64 timing")
     ap.add_argument("--cpu-n", type=int, default=40, help="cube size of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--timing-stride", type=int, default=8,
+    ap.add_argument("--timing-stride", type=int, default=32,
                     help="HIP-event timing of the SpMV kernel on every k-th PCG iteration "
                          "(each timed launch adds an event pair to the stream; 0 disables)")
     ap.add_argument("--no-reference", action="store_true",

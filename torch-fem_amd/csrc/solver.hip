@@ -256,7 +256,16 @@ __global__ void __launch_bounds__(BS) k_spmv_csr(int64_t n, const int64_t *rowpt
 
 // ------------------------------------------------------- preconditioner
 // Inverse of the nodal diagonal blocks (block Jacobi) or of the diagonal
-// (Jacobi) in the inner precision; a zero pivot keeps the identity.
+// (Jacobi) in the inner precision; a zero pivot keeps the identity. The
+// inverse is symmetric and stored packed: the upper triangle, row by row,
+// ND*(ND+1)/2 values per node (pc_index).
+template <int ND> constexpr int pc_size() { return ND * (ND + 1) / 2; }
+
+template <int ND>
+__host__ __device__ constexpr int pc_index(int a, int b) {
+  return a <= b ? a * ND - a * (a - 1) / 2 + (b - a) : b * ND - b * (b - 1) / 2 + (a - b);
+}
+
 template <typename T, int ND>
 __global__ void k_precond(int64_t n_nod, const double *dblk, int pc, T *minv) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -292,8 +301,11 @@ __global__ void k_precond(int64_t n_nod, const double *dblk, int pc, T *minv) {
         }
     }
   }
+  // the two triangles of the inverse agree up to rounding; their mean keeps
+  // the preconditioner exactly symmetric
   for (int a = 0; a < ND; ++a)
-    for (int b = 0; b < ND; ++b) minv[n * ND2 + a * ND + b] = (T)R[a][b];
+    for (int b = a; b < ND; ++b)
+      minv[n * pc_size<ND>() + pc_index<ND>(a, b)] = (T)(0.5 * (R[a][b] + R[b][a]));
 }
 
 template <typename T, int ND, int NRP>
@@ -301,11 +313,12 @@ __device__ __forceinline__ void apply_pc(const T *minv, const T *r, int64_t n, i
                                          T (&z)[ND], T (&rv)[ND]) {
 #pragma unroll
   for (int a = 0; a < ND; ++a) rv[a] = r[(n * ND + a) * NRP + rr];
+  const Vec<T, pc_size<ND>()> mi = ldv<T, pc_size<ND>()>(minv + n * pc_size<ND>());
 #pragma unroll
   for (int a = 0; a < ND; ++a) {
     T acc = T(0);
 #pragma unroll
-    for (int b = 0; b < ND; ++b) acc += minv[n * ND * ND + a * ND + b] * rv[b];
+    for (int b = 0; b < ND; ++b) acc += mi.v[pc_index<ND>(a, b)] * rv[b];
     z[a] = acc;
   }
 }
@@ -578,7 +591,7 @@ __global__ void __launch_bounds__(BSV) k_pcg_update(int64_t n_nod, int it, T *__
     const int64_t n = t / NRP;
     const int rr = (int)(t % NRP);
     const T al = (T)alpha[rr];
-    const Vec<T, ND * ND> mi = ldv<T, ND * ND>(minv + n * ND * ND);
+    const Vec<T, pc_size<ND>()> mi = ldv<T, pc_size<ND>()>(minv + n * pc_size<ND>());
     T rv[ND], zv[ND];
     if constexpr (NRP == 1) {  // the node's ND entries are contiguous
       Vec<T, ND> xv = ldv<T, ND>(x + n * ND), rw = ldv<T, ND>(r + n * ND);
@@ -604,7 +617,7 @@ __global__ void __launch_bounds__(BSV) k_pcg_update(int64_t n_nod, int it, T *__
     for (int a = 0; a < ND; ++a) {
       T z = T(0);
 #pragma unroll
-      for (int b2 = 0; b2 < ND; ++b2) z += mi.v[a * ND + b2] * rv[b2];
+      for (int b2 = 0; b2 < ND; ++b2) z += mi.v[pc_index<ND>(a, b2)] * rv[b2];
       zv[a] = z;
       acc[0] += (double)rv[a] * (double)z;
       acc[1] += (double)rv[a] * (double)rv[a];

@@ -52,6 +52,11 @@ def parse() -> argparse.Namespace:
                          "(each timed launch adds an event pair to the stream; 0 disables)")
     ap.add_argument("--no-reference", action="store_true",
                     help="skip the tight FP64 solve used for the accuracy report")
+    ap.add_argument("--workload", default="hexa1_64",
+                    choices=["hexa1_64", "simp48", "tetra80", "hexa2_32x16"],
+                    help="hexa1_64: the headline Hexa1 64^3 cube (default); the others time the "
+                         "SIMP, thermal and multi-load-case problems the same way (no roofline "
+                         "or CPU baseline)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB buffer 5 times (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -154,8 +159,9 @@ def cpu_model() -> str:
 def cpu_baseline(n: int) -> dict:
     """The CPU oracle (FP64, torch einsum integration, index_add_ assembly,
     SciPy Jacobi-CG at rtol 1e-10, autograd implicit adjoint) on a smaller cube
-    with the same boundary conditions and objective: one warm-up on a tiny
-    cube, then the median of three runs (BASELINE.md's protocol)."""
+    with the same boundary conditions and objective, timed per phase (forward
+    solve, backward) as the reference's benchmarks/utils.py:67-77 does; one
+    warm-up on a tiny cube, then the median of three runs."""
     import statistics
 
     import torch
@@ -199,16 +205,134 @@ def cpu_baseline(n: int) -> dict:
     }
 
 
+def other_workload(args) -> dict:
+    """Mixed-precision and FP64 step times of the other problems of the path,
+    with the errors of the mixed results against an FP64 solve at 1e-11. A
+    step is FEM.solve() (or solve_load_cases) plus the adjoint gradient."""
+    import torch
+
+    from torchfem import Solid, SolidHeat
+    from torchfem.elements import linear_to_quadratic
+    from torchfem.materials import IsotropicConductivity3D, IsotropicElasticity3D
+    from torchfem.mesh import cube_hexa, cube_tetra6
+
+    dt = torch.float64
+    nrhs = 1
+    if args.workload == "simp48":
+        # Hexa1 48^3, rho ~ U(0.3, 1) (seed 0), E = 1000 rho^3, clamped at x=0,
+        # total -z load 1 split equally over the x=1 face; dC/drho of C = F.u
+        nodes, el = cube_hexa(49, 49, 49)
+        rho0 = (0.3 + 0.7 * torch.rand(len(el), generator=torch.Generator().manual_seed(0),
+                                       dtype=dt)).cuda()
+        model = Solid(nodes.cuda(), el.cuda(), IsotropicElasticity3D(1000.0, 0.3))
+        X = model.nodes
+        model.constraints[X[:, 0] == 0.0, :] = True
+        face = X[:, 0] == 1.0
+        model.forces[face, 2] = -1.0 / int(face.sum())
+        F = model.forces.clone()
+        desc = "SIMP Hexa1 48^3, compliance gradient w.r.t. rho"
+
+        def step(precision, stol):
+            rho = rho0.clone().requires_grad_(True)
+            model.material = IsotropicElasticity3D(1000.0 * rho**3, 0.3)
+            u, _, _, _, _ = model.solve(differentiable_parameters=rho, precision=precision,
+                                        stol=stol, preconditioner=args.preconditioner)
+            (g,) = torch.autograd.grad(torch.inner(F.ravel(), u.ravel()), rho)
+            return u.detach(), g
+    elif args.workload == "tetra80":
+        # Tetra1 Kuhn split of 80^3 cells, jitter 0.1 h (seed 0), kappa =
+        # exp(N(0, 0.5^2)) (seed 1), T = 0 at x=0 and 1 at x=1; d(mean T)/d kappa
+        nodes, el = cube_tetra6(80, jitter=0.1, seed=0)
+        k0 = torch.exp(0.5 * torch.randn(len(el), generator=torch.Generator().manual_seed(1),
+                                         dtype=dt)).cuda()
+        model = SolidHeat(nodes.cuda(), el.cuda(), IsotropicConductivity3D(1.0))
+        X = model.nodes
+        model.constraints[X[:, 0] == 0.0] = True
+        model.constraints[X[:, 0] == 1.0] = True
+        model.temperatures[X[:, 0] == 1.0] = 1.0
+        desc = "thermal Tetra1 80^3 x 6, gradient of mean T w.r.t. kappa"
+
+        def step(precision, stol):
+            kap = k0.clone().requires_grad_(True)
+            model.material = IsotropicConductivity3D(kap)
+            T, _, _, _, _ = model.solve(differentiable_parameters=kap, precision=precision,
+                                        stol=stol, preconditioner=args.preconditioner)
+            (g,) = torch.autograd.grad(T.mean(), kap)
+            return T.detach(), g
+    else:
+        # Hexa2 32^3, clamped at z=0, 16 load cases: tractions ~N(0,1) per face
+        # of z=1 (seed k) as consistent nodal loads of unit total magnitude;
+        # one batched solve and one batched adjoint of sum_k C_k w.r.t. E
+        nodes, el = linear_to_quadratic(*cube_hexa(33, 33, 33))
+        model = Solid(nodes.cuda(), el.cuda(), IsotropicElasticity3D(1000.0, 0.3))
+        X = model.nodes
+        model.constraints[X[:, 2] == 0.0, :] = True
+        top = X[:, 2] == 1.0
+        loads = []
+        for k in range(16):
+            trac = torch.randn(32 * 32, 3, generator=torch.Generator().manual_seed(k), dtype=dt)
+            f = model.integrate_surface_load(top, trac.cuda())
+            loads.append(f / f.sum(0).norm())
+        Fb = torch.stack(loads)
+        nrhs = 16
+        desc = "Hexa2 32^3, 16 load cases, batched solve + adjoint of sum_k C_k w.r.t. E"
+
+        def step(precision, stol):
+            E = torch.full((model.n_elem,), 1000.0, dtype=dt, device="cuda", requires_grad=True)
+            model.material = IsotropicElasticity3D(E, 0.3)
+            u, _ = model.solve_load_cases(Fb, differentiable_parameters=E, precision=precision,
+                                          stol=stol, preconditioner=args.preconditioner)
+            (g,) = torch.autograd.grad((Fb * u).sum(), E)
+            return u.detach(), g
+
+    log(f"{desc}: {model.n_dofs} DOFs, {nrhs} load case(s)")
+    ms_mixed, (u_mix, g_mix) = timed(lambda: step("mixed", args.stol), args.steps, args.warmup,
+                                     "mixed")
+    info = dict(model.last_solve_info)
+    ms_fp64 = None
+    if args.fp64_steps > 0:
+        ms_fp64, _ = timed(lambda: step("fp64", args.stol), args.fp64_steps, 1, "fp64")
+    acc = {}
+    if not args.no_reference:
+        u_ref, g_ref = step("fp64", 1e-11)
+        rel = lambda a, b: float((a - b).norm() / b.norm())  # noqa: E731
+        acc = {"u_rel_err_vs_fp64": rel(u_mix, u_ref), "grad_rel_err_vs_fp64": rel(g_mix, g_ref)}
+    fwd = info["forward"]["pcg"]
+    return {
+        "metric": f"solve+adjoint-grad time & DOFs/s, {desc}, mixed-prec vs FP64 @{args.stol:g} rel",
+        "value": model.n_dofs * nrhs / ms_mixed,
+        "unit": "DOFs/s (per load case, summed)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_mixed * 1e3,
+        "higher_is_better": True,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "n_dofs": model.n_dofs, "load_cases": nrhs,
+                   "nnz": model._pattern.nnz, "stol": args.stol,
+                   "preconditioner": args.preconditioner},
+        "fp64_ms_per_step": ms_fp64 * 1e3 if ms_fp64 else None,
+        "speedup_vs_fp64": ms_fp64 / ms_mixed if ms_fp64 else None,
+        "pcg_iterations": {"mixed_fwd": [sum(p["iterations"][r] for p in fwd) for r in range(nrhs)],
+                           "mixed_adj": info["adjoint"]["iterations"][:nrhs]},
+        "accuracy": acc,
+    }
+
+
 def main() -> None:
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding is "
-              "described in DESIGN.md but not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); sharding the path over "
+              "several GPUs is not built", file=sys.stderr)
         sys.exit(2)
     import torch
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.workload != "hexa1_64":
+        print(json.dumps(other_workload(args)), flush=True)
         return
     log(f"building Hexa1 cube {args.n}^3")
     model, setup = build_model(args.n)
@@ -316,9 +440,9 @@ def main() -> None:
         "roofline": roof,
         "cpu_baseline": None,
         "published_context": {
-            "note": "no published number for this metric (BASELINE.json published = {}); the "
-                    "nearest reference rows are FP64 solve()+backward() of its cube benchmark "
-                    "(load u_x=0.1 at x=1, clamped x=0), DOFs/s derived as dofs/(fwd+bwd)",
+            "note": "no published number for this metric; the nearest reference rows are "
+                    "FP64 solve()+backward() of its cube benchmark (load u_x=0.1 at x=1, "
+                    "clamped x=0), DOFs/s derived as dofs/(fwd+bwd)",
             "rtx5090_amgx_cube_n60_dofs_per_s": 220957,
             "rtx5090_amgx_cube_n70_dofs_per_s": 237441,
             "threadripper_9975wx_cube_n60_dofs_per_s": 43461,

@@ -213,6 +213,84 @@ __device__ __forceinline__ void node_product(const tfm_mesh_t &m, const T *__res
   node_product_at<T, ND, TL, NRP, D16>(m, val, x, n, node_ptr(m, n), lane, acc);
 }
 
+// Several right-hand sides (NRP >= 4): one 64-lane wave per node. Lane
+// l = q*NRP + 4*g + t (t < 4) works on rhs group g (rhs 4g..4g+3) and on the
+// neighbour blocks p = s, s + S, ... of slice s = 4*q + t (S = 256/NRP
+// slices). The lanes of one slice load the same block rows (one transaction
+// serves them) and the four rhs of their group at the neighbour in one
+// 16-byte load (contiguous in the interleaved layout). A butterfly over the
+// slices (lane offsets 1, 2 and multiples of NRP) leaves the ND x 4 sums of
+// group g in every lane of the group, so lane l holds entry e = l of the
+// node's ND*NRP outputs (rhs l % NRP). ND*4 accumulators per lane instead of
+// ND*NRP keep the kernel's occupancy.
+template <int TL, int NRP> constexpr bool rhs_lanes() { return TL == 64 && NRP >= 4; }
+
+template <typename T, int ND, int NRP, bool D16>
+__device__ __forceinline__ void node_product_mr(const tfm_mesh_t &m, const T *__restrict__ val,
+                                                const T *__restrict__ x, int64_t n,
+                                                const NodePtr &np, int lane, T (&acc)[ND][4]) {
+  constexpr int S = 256 / NRP;
+  const int j = lane % NRP, g = j / 4;
+  const int sl = (lane / NRP) * 4 + (j % 4);
+  auto nbr = [&](int k) -> int64_t { return D16 ? n + (int64_t)m.adj16[k] : (int64_t)m.adj[k]; };
+#pragma unroll
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[a][t] = T(0);
+  const T *vn = val + np.v0;
+  auto block = [&](int p, int64_t c) {
+    Vec<T, ND> rows[ND];
+    Vec<T, 4> xv[ND];
+#pragma unroll
+    for (int a = 0; a < ND; ++a) rows[a] = ldv<T, ND>(vn + a * ND * np.deg + p * ND);
+#pragma unroll
+    for (int b = 0; b < ND; ++b) xv[b] = ldv<T, 4>(x + (c * ND + b) * NRP + 4 * g);
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int b = 0; b < ND; ++b)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[a][t] += rows[a].v[b] * xv[b].v[t];
+  };
+  if (np.deg > 0) {
+    int p = sl;
+    for (; p + S < np.deg; p += 2 * S) {  // two blocks in flight per lane
+      const int64_t c0 = nbr(np.a0 + p), c1 = nbr(np.a0 + p + S);
+      block(p, c0);
+      block(p + S, c1);
+    }
+    if (p < np.deg) block(p, nbr(np.a0 + p));
+  } else if (sl == 0) {  // orphan node: only its diagonal is stored
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[a][t] = vn[a] * x[(n * ND + a) * NRP + 4 * g + t];
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    if (off >= 4 && off < NRP) continue;  // these offsets change the rhs group
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[a][t] += __shfl_xor(acc[a][t], off, 64);
+  }
+}
+
+// Entry e = a*NRP + r of a node's outputs from the group sums of the lane
+// that holds rhs r (a select chain, no indexed registers).
+template <typename T, int ND, int NRP>
+__device__ __forceinline__ T pick_entry(const T (&acc)[ND][4], int e) {
+  const int a = e / NRP, t = e % 4;
+  T w = acc[0][0];
+#pragma unroll
+  for (int aa = 0; aa < ND; ++aa)
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt)
+      if (aa == a && tt == t) w = acc[aa][tt];
+  return w;
+}
+
+
 // y = K x over the node blocks (every right-hand side of the interleaved x).
 template <typename T, int ND, int TL, int NRP, bool D16>
 __global__ void __launch_bounds__(BS) k_spmv(tfm_mesh_t m, const T *__restrict__ val,
@@ -220,13 +298,19 @@ __global__ void __launch_bounds__(BS) k_spmv(tfm_mesh_t m, const T *__restrict__
   constexpr int TPB = BS / TL;
   const int lane = threadIdx.x % TL, team = threadIdx.x / TL;
   for (int64_t n = (int64_t)blockIdx.x * TPB + team; n < m.n_nod; n += (int64_t)gridDim.x * TPB) {
-    T acc[ND][NRP];
-    node_product<T, ND, TL, NRP, D16>(m, val, x, n, lane, acc);
+    if constexpr (rhs_lanes<TL, NRP>()) {
+      T am[ND][4];
+      node_product_mr<T, ND, NRP, D16>(m, val, x, n, node_ptr(m, n), lane, am);
+      if (lane < ND * NRP) y[n * ND * NRP + lane] = pick_entry<T, ND, NRP>(am, lane);
+    } else {
+      T acc[ND][NRP];
+      node_product<T, ND, TL, NRP, D16>(m, val, x, n, lane, acc);
 #pragma unroll
-    for (int a = 0; a < ND; ++a)
+      for (int a = 0; a < ND; ++a)
 #pragma unroll
-      for (int r = 0; r < NRP; ++r)
-        if ((a * NRP + r) % TL == lane) y[(n * ND + a) * NRP + r] = acc[a][r];
+        for (int r = 0; r < NRP; ++r)
+          if ((a * NRP + r) % TL == lane) y[(n * ND + a) * NRP + r] = acc[a][r];
+    }
   }
 }
 
@@ -485,18 +569,25 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
         sp[k] = s[nb + e];
       }
     }
-    T acc[ND][NRP];
-    node_product_at<T, ND, TL, NRP, D16>(m, val, u, n, cur, lane, acc);
+    T acc[ND][NRP], am[ND][4];
+    if constexpr (rhs_lanes<TL, NRP>())
+      node_product_mr<T, ND, NRP, D16>(m, val, u, n, cur, lane, am);
+    else
+      node_product_at<T, ND, TL, NRP, D16>(m, val, u, n, cur, lane, acc);
 #pragma unroll
     for (int k = 0; k < KE; ++k) {
       const int e = lane + k * TL;
       if (lv && e < NE) {
         T w = T(0);
+        if constexpr (rhs_lanes<TL, NRP>()) {
+          w = pick_entry<T, ND, NRP>(am, e);  // e = lane: rhs lane % NRP
+        } else {
 #pragma unroll
-        for (int a = 0; a < ND; ++a)
+          for (int a = 0; a < ND; ++a)
 #pragma unroll
-          for (int r = 0; r < NRP; ++r)
-            if (a * NRP + r == e) w = acc[a][r];
+            for (int r = 0; r < NRP; ++r)
+              if (a * NRP + r == e) w = acc[a][r];
+        }
         const T pv = up[k] + b * pp[k];
         const T sv = w + b * sp[k];
         p[nb + e] = pv;
@@ -1005,11 +1096,16 @@ static SpmvShape spmv_shape(const tfm_mesh_t &m, int nrp) {
     default: TFM_REQUIRE(false, "ndof must be 1, 2 or 3");                                     \
   }
 
-// Team width TL (16 or 32; a multiple of NRP <= 16).
+// Team width TL: 16 or 32 lanes per node, or a whole wave with the rhs
+// across lanes (node_product_mr) for FP64 with four or more right-hand sides,
+// where a lane cannot hold ND*NRP doubles without losing occupancy. (In FP32
+// the lane-per-block product measured faster for 16 rhs: 274 against 307 us
+// on Hexa2 32^3.) T and NRP must be in scope.
+#define TFM_WIDE_TEAM (sizeof(T) == 8 && NRP >= 4)
 #define TFM_TEAM_SWITCH(tl, ...)                                                               \
-  switch (tl) {                                                                                \
-    case 16: { constexpr int TL = 16; __VA_ARGS__; break; }                                    \
-    default: { constexpr int TL = 32; __VA_ARGS__; break; }                                    \
+  switch (TFM_WIDE_TEAM ? 64 : (tl)) {                                                         \
+    case 16: { constexpr int TL = TFM_WIDE_TEAM ? 64 : 16; __VA_ARGS__; break; }               \
+    default: { constexpr int TL = TFM_WIDE_TEAM ? 64 : 32; __VA_ARGS__; break; }               \
   }
 
 template <typename T, int ND, int NRP>
@@ -1136,7 +1232,9 @@ static int pcg_alloc(tfm_pcg *h) {
   TFM_CHECK_HIP(hipMalloc(&h->rzold, sizeof(double) * nrp));
   // one eta partial set per block of A; its widest grid has one node per
   // 8-lane share of a block, at most n / 8 + 1 blocks
-  h->pq_cap = (int)std::max<int64_t>(RBA, n / 8 + 1);
+  // one partial set per block of the full-grid A: at least 4 nodes per block
+  // (the 64-lane team), n / 4 covers it for one DOF per node
+  h->pq_cap = (int)std::max<int64_t>(RBA, n / 4 + 1);
   TFM_CHECK_HIP(hipMalloc(&h->part_pq, sizeof(double) * (size_t)h->pq_cap * nrp));
   TFM_CHECK_HIP(hipMalloc(&h->part_upd, sizeof(double) * RBW * 2 * nrp));
   TFM_CHECK_HIP(hipMalloc(&h->part_aux, sizeof(double) * RBW * nrp));

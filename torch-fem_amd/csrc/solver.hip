@@ -146,7 +146,7 @@ __device__ __forceinline__ NodePtr node_ptr(const tfm_mesh_t &m, int64_t n) {
   return NodePtr{a0, m.adj_ptr[n + 1] - a0, m.val_ptr[n]};
 }
 
-template <typename T, int ND, int TL, int NRP, bool D16>
+template <typename T, int ND, int TL, int NRP, bool D16, bool BFLY = true>
 __device__ __forceinline__ void node_product_at(const tfm_mesh_t &m, const T *__restrict__ val,
                                                 const T *__restrict__ x, int64_t n,
                                                 const NodePtr &np, int lane, T (&acc)[ND][NRP]) {
@@ -198,12 +198,55 @@ __device__ __forceinline__ void node_product_at(const tfm_mesh_t &m, const T *__
 #pragma unroll
         for (int r = 0; r < NRP; ++r) acc[a][r] = vn[a] * x[(n * ND + a) * NRP + r];
   }
+  if constexpr (BFLY) {
 #pragma unroll
-  for (int off = 1; off < TL; off <<= 1)
+    for (int off = 1; off < TL; off <<= 1)
 #pragma unroll
-    for (int a = 0; a < ND; ++a)
+      for (int a = 0; a < ND; ++a)
 #pragma unroll
-      for (int r = 0; r < NRP; ++r) acc[a][r] += __shfl_xor(acc[a][r], off, TL);
+        for (int r = 0; r < NRP; ++r) acc[a][r] += __shfl_xor(acc[a][r], off, TL);
+  }
+}
+
+// FP32 with four or more rhs: the team's ND*NRP sums per lane are reduced
+// through LDS instead of a butterfly (48 values over 5 butterfly steps cost
+// 240 lane exchanges per node for 16 rhs). Each lane writes its sums as one
+// row of a TL x (ND*NRP) table (padded to an odd row length, so neither
+// the row writes nor the column reads conflict), then lane l adds column
+// e = l + k*TL over the TL rows in order. outv[k] is entry e of the node's
+// outputs. The team is inside one wave, whose LDS accesses complete in
+// order, so only the compiler has to be kept from reordering them.
+template <typename T, int TL, int NRP> constexpr bool lds_reduce() {
+  return sizeof(T) == 4 && NRP >= 4 && TL < 64;
+}
+
+template <typename T, int ND, int TL, int NRP, bool D16>
+__device__ __forceinline__ void node_product_lds(const tfm_mesh_t &m, const T *__restrict__ val,
+                                                 const T *__restrict__ x, int64_t n,
+                                                 const NodePtr &np, int lane, int team,
+                                                 T (&outv)[(ND * NRP + TL - 1) / TL]) {
+  constexpr int NE = ND * NRP, KE = (NE + TL - 1) / TL, PAD = NE + 1;
+  __shared__ T red[BS / TL][TL][PAD];
+  T acc[ND][NRP];
+  node_product_at<T, ND, TL, NRP, D16, false>(m, val, x, n, np, lane, acc);
+#pragma unroll
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int r = 0; r < NRP; ++r) red[team][lane][a * NRP + r] = acc[a][r];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int k = 0; k < KE; ++k) {
+    const int e = lane + k * TL;
+    T t = T(0);
+    if (e < NE)
+      for (int q = 0; q < TL; ++q) t += red[team][q][e];
+    outv[k] = t;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <typename T, int ND, int TL, int NRP, bool D16>
@@ -302,6 +345,13 @@ __global__ void __launch_bounds__(BS) k_spmv(tfm_mesh_t m, const T *__restrict__
       T am[ND][4];
       node_product_mr<T, ND, NRP, D16>(m, val, x, n, node_ptr(m, n), lane, am);
       if (lane < ND * NRP) y[n * ND * NRP + lane] = pick_entry<T, ND, NRP>(am, lane);
+    } else if constexpr (lds_reduce<T, TL, NRP>()) {
+      constexpr int KE = (ND * NRP + TL - 1) / TL;
+      T outv[KE];
+      node_product_lds<T, ND, TL, NRP, D16>(m, val, x, n, node_ptr(m, n), lane, team, outv);
+#pragma unroll
+      for (int k = 0; k < KE; ++k)
+        if (lane + k * TL < ND * NRP) y[n * ND * NRP + lane + k * TL] = outv[k];
     } else {
       T acc[ND][NRP];
       node_product<T, ND, TL, NRP, D16>(m, val, x, n, lane, acc);
@@ -569,9 +619,11 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
         sp[k] = s[nb + e];
       }
     }
-    T acc[ND][NRP], am[ND][4];
+    T acc[ND][NRP], am[ND][4], outv[KE];
     if constexpr (rhs_lanes<TL, NRP>())
       node_product_mr<T, ND, NRP, D16>(m, val, u, n, cur, lane, am);
+    else if constexpr (lds_reduce<T, TL, NRP>())
+      node_product_lds<T, ND, TL, NRP, D16>(m, val, u, n, cur, lane, team, outv);
     else
       node_product_at<T, ND, TL, NRP, D16>(m, val, u, n, cur, lane, acc);
 #pragma unroll
@@ -581,6 +633,8 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
         T w = T(0);
         if constexpr (rhs_lanes<TL, NRP>()) {
           w = pick_entry<T, ND, NRP>(am, e);  // e = lane: rhs lane % NRP
+        } else if constexpr (lds_reduce<T, TL, NRP>()) {
+          w = outv[k];
         } else {
 #pragma unroll
           for (int a = 0; a < ND; ++a)

@@ -1123,7 +1123,10 @@ static SpmvShape spmv_shape(const tfm_mesh_t &m, int nrp) {
   const bool fp64 = sizeof(T) == 8;
   sh.tl = (fp64 && deg > 16.0) || deg > 32.0 ? 32 : 16;
   sh.scal = fp64;
-  sh.cap = fp64 ? (1 << 30) : (nrp <= 2 ? RBA : nrp <= 8 ? RBA / 2 : RBA / 4);
+  // FP64 with one or two rhs: the full grid (one node per team); with more,
+  // a capped grid-stride grid, so that the one-block scalar kernel sums few
+  // partial sets (35k blocks x 16 rhs of Hexa2 32^3 took it 84 us)
+  sh.cap = fp64 ? (nrp >= 4 ? RBA / 2 : (1 << 30)) : (nrp <= 2 ? RBA : nrp <= 8 ? RBA / 2 : RBA / 4);
   static const int g_env = env_int("TFM_SPMV_GRID", 0);
   static const int f_env = env_int("TFM_SPMV_SCAL", -1);
   if (g_env > 0) sh.cap = g_env;
@@ -1402,10 +1405,11 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
               h->coef + 1, xcd};
   const unsigned g1 = (unsigned)((n + 255) / 256);
   // vector kernels: one (node, rhs) per thread when a scalar kernel sums
-  // their partials, else at most RBV blocks for the prologue of A
+  // their partials (at most RBW/NRP blocks, so that it sums few partial
+  // sets), else at most RBV blocks for the prologue of A
   static const int vcap = std::min(RBW, std::max(1, env_int("TFM_VEC_GRID", RBV)));
   // whole multiples of the CU count, so that no CU runs a second round alone
-  const unsigned gv = std::min<unsigned>(ops.scal ? RBW : vcap,
+  const unsigned gv = std::min<unsigned>(ops.scal ? std::max(RBV, RBW / NRP) : vcap,
                                          round_to_cus(grid_for(n_nod * NRP, BSV, RBW)));
   std::vector<double> htol(NRP, -1.0), bnorm(NRP, 0.0), rprev(NRP, 0.0), rnow(NRP, 0.0);
   std::vector<int> fin(NRP, 1), stall(NRP, 0);

@@ -234,9 +234,12 @@ int tfm_pcg_destroy(tfm_pcg_t *h);
  * next FP64 refinement, and the segment that can end the solve aims at
  * final_rtol ||b||_2 (0 < final_rtol <= rtol; a margin below rtol for the error
  * of the solution); FP64 mode aims at 0.7 rtol.
+ * A right-hand side with b = 0 gets x = 0 whatever the initial guess (as
+ * scipy's cg/minres return for ||b|| = 0).
  * iters[nrhs] receives the total inner iterations, relres[nrhs] the final FP64
  * relative residual, outer[0] the refinement steps (mixed mode).
- * Returns TFM_ERR_NOCONV when max_iter is exhausted. */
+ * Returns TFM_ERR_NOCONV when max_iter is exhausted, or when the true residual
+ * of a mixed solve stops falling above rtol (attainable accuracy reached). */
 int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t s, const tfm_system_t *sys,
                   int32_t prec, int32_t pc, const double *b, double *x,
                   int32_t nrhs, double rtol, double inner_rtol, double final_rtol,

@@ -72,7 +72,7 @@ __device__ __forceinline__ void write_block_partials(const double (&v)[NV], doub
 // in a fixed order, so that every block that calls it gets the same bits.
 // Thread t adds the partials b = t/NVT + k*CH of slot t % NVT with four
 // loads in flight; a wave butterfly and one LDS stage over the waves follow
-// (a serial chain through LDS here used to cost microseconds per kernel).
+// (one thread adding a 256-long LDS column serially costs microseconds).
 template <int NVT, int BSZ = BS>
 __device__ __forceinline__ void sum_partials(const double *part, int nblk, double *sh,
                                              double *out) {
@@ -576,8 +576,8 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
   int64_t n = sp_.first + team;
   // grid-stride mode: row pointers run one node ahead of the product, so
   // each node's loads start from pointers already in registers (the first
-  // ones load during the prologue). The full grid of the CO mode has one
-  // node per team and loads them in place.
+  // ones load during the prologue). The CO mode, mostly one node per team
+  // on the full grid, loads them in place.
   NodePtr np{0, 0, 0};
   if constexpr (!CO) {
     if (n < n_end) np = node_ptr(m, n);

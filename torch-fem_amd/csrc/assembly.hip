@@ -422,8 +422,22 @@ __global__ void k_gather_nodes(tfm_mesh_t m, const double *fe, double *F) {
   const int64_t n = t / NDOF;
   const int a = (int)(t - n * NDOF);
   double acc = 0.0;
-  for (int k = m.ne_ptr[n]; k < m.ne_ptr[n + 1]; ++k)
-    acc += fe[((int64_t)m.ne_elem[k] * m.npe + m.ne_loc[k]) * NDOF + a];
+  int k = m.ne_ptr[n];
+  const int k_end = m.ne_ptr[n + 1];
+  auto at = [&](int i) -> int64_t {
+    return ((int64_t)m.ne_elem[i] * m.npe + m.ne_loc[i]) * NDOF + a;
+  };
+  // four incident elements per round: their index and value loads are in
+  // flight together, and the sum keeps the ascending element order
+  for (; k + 3 < k_end; k += 4) {
+    const int64_t i0 = at(k), i1 = at(k + 1), i2 = at(k + 2), i3 = at(k + 3);
+    const double v0 = fe[i0], v1 = fe[i1], v2 = fe[i2], v3 = fe[i3];
+    acc += v0;
+    acc += v1;
+    acc += v2;
+    acc += v3;
+  }
+  for (; k < k_end; ++k) acc += fe[at(k)];
   F[t] = acc;
 }
 

@@ -139,3 +139,23 @@ def test_batch_with_a_zero_load_case(precision):
         model.heat_flux = Fb[k]
         Tk, _, _, _, _ = model.solve(precision="fp64", stol=1e-12)
         assert rel(T[k], Tk) < 1e-8
+
+
+@pytest.mark.parametrize("ncase", [3, 6])
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_batched_tetra_cases_match_single_solves(precision, ncase):
+    """Batched solves on a Tetra1 mesh (about 15 neighbours per node: the
+    8-lane teams, with four and eight padded rhs) agree with single solves."""
+    nodes, el = cube_tetra6(5, jitter=0.1, seed=5)
+    model = Solid(nodes.cuda(), el.cuda(), IsotropicElasticity3D(1000.0, 0.3))
+    X = model.nodes
+    model.constraints[X[:, 2] == 0.0, :] = True
+    g = torch.Generator().manual_seed(6)
+    Fb = torch.randn(ncase, len(nodes), 3, generator=g, dtype=DT).cuda()
+    Fb[:, X[:, 2] == 0.0] = 0.0
+    u, f = model.solve_load_cases(Fb, precision=precision, stol=1e-10)
+    for k in range(ncase):
+        model.forces = Fb[k]
+        uk, fk, _, _, _ = model.solve(precision="fp64", stol=1e-12)
+        assert rel(u[k], uk) < 1e-8
+        assert rel(f[k], fk) < 1e-8

@@ -1481,7 +1481,7 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
   // Batches between host checks: the first of a segment has CHK iterations;
   // later ones are sized from the contraction of rr observed so far, to end
   // the segment with few empty launches and few host round trips.
-  const int CHK = 16, CHK_MIN = 4, CHK_MAX = 64;
+  const int CHK = 12, CHK_MIN = 4, CHK_MAX = 64;
   int it = 0, n_outer = 0, seg_first = 0;
   bool any_live = mask != 0;
   while (any_live) {

@@ -120,7 +120,7 @@ typedef struct tfm_system {
   const tfm_material_t *mat;
   const uint8_t *con;        /* [n_nod*ndof] 1 where the DOF is constrained */
   const double *val64;       /* FP64 CSR values or NULL */
-  const float *val32;        /* FP32 CSR values or NULL */
+  const float *val32;        /* FP32 CSR values in difference form (see tfm_assemble) or NULL */
   const double *diag_blocks; /* [n_nod, ndof, ndof] nodal diagonal blocks of K */
 } tfm_system_t;
 
@@ -166,7 +166,12 @@ int tfm_element_matrices(tfm_stream_t s, const tfm_mesh_t *m,
 /* Fused integration + deterministic assembly + Dirichlet epilogue: writes the
  * CSR values of K in FP64 and/or FP32 (either may be NULL) and the nodal
  * diagonal blocks. Replaces integrate_material's k plus
- * FEM.assemble_matrix (base.py:398-426) with no float atomics. */
+ * FEM.assemble_matrix (base.py:398-426) with no float atomics.
+ * The FP32 values are the FP64 ones rounded, except in each node's diagonal
+ * block, which holds the rounded FP64 row-sum block R_n = sum_j K_nj (the
+ * "difference form": the FP32 products compute
+ * sum_{j != n} K_nj (x_j - x_n) + R_n x_n, which equals K x exactly and
+ * keeps the FP32 operator close to the FP64 one on smooth vectors). */
 int tfm_assemble(tfm_stream_t s, const tfm_mesh_t *m, const tfm_material_t *mat,
                  const uint8_t *con, double *val64, float *val32,
                  double *diag_blocks, int32_t *bad_jac);
@@ -210,7 +215,9 @@ int tfm_sensitivity(tfm_stream_t s, const tfm_mesh_t *m, const tfm_material_t *m
                     double *out);
 
 /* ----------------------------------------------------- sparse kernels */
-/* y = K x on the node-block CSR values, FP64 (prec 0) or FP32 (prec 1). */
+/* y = K x on the node-block CSR values, FP64 (prec 0) or FP32 (prec 1; the
+ * values in the difference form that tfm_assemble writes, products summed
+ * in FP64 for one or two right-hand sides). */
 int tfm_spmv(tfm_stream_t s, const tfm_mesh_t *m, int32_t prec, const void *vals,
              const void *x, void *y);
 
@@ -258,6 +265,51 @@ int tfm_csr_pcg_solve(tfm_pcg_t *h, tfm_stream_t s, int64_t n,
                       const int64_t *rowptr, const int32_t *col,
                       const double *val, const double *b, double *x, double rtol,
                       int32_t *iters, double *relres);
+
+/* ------------------------------------------------------- distributed loads */
+/* Facets per element (*nf) and nodes per facet (*nfn) of the mesh's element
+ * type (the facet tables of src/torchfem/elements.py). */
+int tfm_facet_count(const tfm_mesh_t *m, int32_t *nf, int32_t *nfn);
+
+/* FEM._boundary_facets (base.py:452-470): the facets whose nodes are all set
+ * in mask [n_nod] and that belong to one element only, in the order of their
+ * sorted node tuples (the reference's torch.unique order). Work arrays:
+ * flag [n_elem*nf] int8, off [n_nod+1], bsum [ceil(n_nod/1024)+1].
+ * Outputs: pos [n_elem*nf] (the facet's index in the list, or -1),
+ * list [list_cap] (slot e*nf + local facet of each listed facet) and
+ * *n_facets (host). Synchronises the stream once to read the count; fails
+ * with TFM_ERR_ARG if it exceeds list_cap (pos is complete then, list not). */
+int tfm_boundary_facets(tfm_stream_t s, const tfm_mesh_t *m, const uint8_t *mask,
+                        int8_t *flag, int32_t *pos, int32_t *off, int32_t *bsum,
+                        int32_t *list, int64_t list_cap, int64_t *n_facets);
+
+/* Node ids [n_facets, nfn] (int64, the element's winding) of listed facets. */
+int tfm_facet_nodes(tfm_stream_t s, const tfm_mesh_t *m, const int32_t *list,
+                    int64_t n_facets, int64_t *out);
+
+/* FEM._integrate_facet_load (base.py:472-494) on the listed facets: consistent
+ * nodal loads F [n_nod, k] = sum_gp w N detJ t, detJ = sqrt(det(J J^T)).
+ * kind 0: pressure load[0] along the unit outward normal (k = dim);
+ * kind 1: one traction load[k] for every facet; kind 2: load [n_facets, k].
+ * fc [n_facets, nfn, k] receives the facet contributions, which are summed
+ * at each node over its incident elements in ascending order. */
+int tfm_facet_load(tfm_stream_t s, const tfm_mesh_t *m, const uint8_t *mask,
+                   const int32_t *pos, const int32_t *list, int64_t n_facets,
+                   const double *load, int32_t kind, int32_t k, double *fc, double *F);
+
+/* FEM.integrate_body_load (base.py:496-513): F [n_nod, k] from a load per
+ * unit volume, load[k] for every element (per_element 0) or [n_elem, k],
+ * times scale [n_elem] (planar thickness; NULL = 1). fe [n_elem, npe, k] is
+ * work space. *bad_jac is set to 1 if some detJ <= 0. */
+int tfm_body_load(tfm_stream_t s, const tfm_mesh_t *m, const double *scale,
+                  const double *load, int32_t per_element, int32_t k, double *fe,
+                  double *F, int32_t *bad_jac);
+
+/* FEM.eval_shape_functions (base.py:293-314) at nq local points xi [nq, dim]:
+ * B [nq, n_elem, dim, npe] = J^{-1} dN/dxi and detJ [nq, n_elem].
+ * *bad_jac is set to 1 if some detJ <= 0. */
+int tfm_shape_gradients(tfm_stream_t s, const tfm_mesh_t *m, const double *xi, int32_t nq,
+                        double *B, double *detJ, int32_t *bad_jac);
 
 #ifdef __cplusplus
 }

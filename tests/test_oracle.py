@@ -234,6 +234,51 @@ def test_heat_k0_symmetric_zero_row_sums_scales():
     assert torch.allclose(m2.k0(), 2 * k)
 
 
+def test_heat_unit_tetrahedron_closed_form():
+    """The unit tetrahedron (0,0,0), (1,0,0), (0,1,0), (0,0,1) has the constant
+    shape gradients g0 = (-1,-1,-1), g1 = e1, g2 = e2, g3 = e3 and volume 1/6,
+    so its conduction matrix is k_ab = (1/6) g_a . KAPPA g_b exactly. A wrong
+    factor (volume, weight, KAPPA scaling or transposition) fails this."""
+    nodes = torch.tensor([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]], dtype=DT)
+    el = torch.tensor([[0, 1, 2, 3]])
+    g = torch.tensor([[-1.0, -1.0, -1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]], dtype=DT)
+    for KAP in (2.5 * torch.eye(3, dtype=DT),
+                torch.tensor([[2.0, 0.3, -0.2], [0.3, 1.5, 0.1], [-0.2, 0.1, 1.2]], dtype=DT)):
+        m = O.OracleModel(nodes, el, O.Tetra1, "heat", KAP[None])
+        k = m.k0()[0]
+        assert torch.allclose(k, g @ KAP @ g.T / 6.0, atol=1e-14)
+    # isotropic: (kappa / 6) [[3,-1,-1,-1],[-1,1,0,0],[-1,0,1,0],[-1,0,0,1]]
+    m = O.OracleModel(nodes, el, O.Tetra1, "heat", O.iso_KAPPA(torch.tensor([6.0], dtype=DT)))
+    expect = torch.tensor([[3.0, -1, -1, -1], [-1, 1, 0, 0], [-1, 0, 1, 0], [-1, 0, 0, 1]], dtype=DT)
+    assert torch.allclose(m.k0()[0], expect, atol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["Tetra1", "Hexa1", "Hexa2"])
+def test_heat_patch_test_flux(name):
+    """A linear temperature T = a.x + c prescribed on the boundary of a body of
+    constant (anisotropic) conductivity is reproduced inside, and every
+    Gauss point carries the flux KAPPA a (conductivity.py:90-92) with zero
+    net heat at interior nodes. Pins the heat integration up to every factor
+    that a symmetric, zero-row-sum matrix could hide."""
+    if name == "Tetra1":
+        nodes, el = O.cube_tetra6(3, jitter=0.15, seed=3)
+    elif name == "Hexa1":
+        nodes, el = O.cube_hexa(4, 4, 4)
+    else:
+        nodes, el = O.linear_to_quadratic(*O.cube_hexa(3, 3, 3))
+    KAP = torch.tensor([[2.0, 0.3, -0.2], [0.3, 1.5, 0.1], [-0.2, 0.1, 1.2]], dtype=DT)
+    m = O.OracleModel(nodes, el, O.ELEMENTS[name], "heat", KAP.expand(len(el), 3, 3))
+    a = torch.tensor([0.7, -0.4, 0.25], dtype=DT)
+    T_exact = (nodes @ a + 0.1)[:, None]
+    bnd = _boundary(nodes)
+    T, f, flux, _ = m.solve(bnd[:, None], torch.zeros_like(T_exact), T_exact)
+    assert torch.allclose(T, T_exact, atol=1e-12)
+    assert torch.allclose(flux, (KAP @ a).expand(len(el), 1, 3), atol=1e-12)
+    assert f[~bnd].abs().max() < 1e-12
+    # the net boundary heat balances: sum of reactions = 0
+    assert abs(float(f.sum())) < 1e-12
+
+
 # ---------------------------------------------------------------- solves
 def _boundary(nodes):
     mask = torch.zeros(len(nodes), dtype=torch.bool)

@@ -293,9 +293,21 @@ __global__ void __launch_bounds__(64) k_assemble_rows(tfm_mesh_t m, tfm_material
     const int64_t col = (int64_t)m.adj[a0 + p] * NDOF + b;
     double v = acc[q];
     if (con[row] || con[col]) v = (row == col) ? 1.0 : 0.0;
+    acc[q] = v;
     if (v64) v64[vb + q] = v;
-    if (v32) v32[vb + q] = (float)v;
+    if (v32 && p != pself) v32[vb + q] = (float)v;
     if (p == pself) dblk[n * ND2 + a * NDOF + b] = v;
+  }
+  // FP32 difference form: the diagonal block holds the row sums of the
+  // final FP64 values, added in ascending slot order
+  if (v32) {
+    __syncthreads();
+    if (tid < ND2) {
+      const int a = tid / NDOF, b = tid - a * NDOF;
+      double r = 0.0;
+      for (int p = 0; p < deg; ++p) r += acc[a * L + p * NDOF + b];
+      v32[vb + a * L + pself * NDOF + b] = (float)r;
+    }
   }
 }
 
@@ -480,8 +492,8 @@ __global__ void __launch_bounds__(64) k_assemble_elements(tfm_mesh_t m, const do
   for (int p = 0; p < deg; ++p)
     if (m.adj[a0 + p] == n) pself = p;
   const int L = NDOF * deg;
-  for (int q = tid; q < ND2 * deg; q += 64) {
-    const int a = q / L, rem = q - a * L, p = rem / NDOF, b = rem - p * NDOF;
+  // entry (a, p, b) of the node's rows with the Dirichlet epilogue applied
+  auto entry = [&](int a, int p, int b) -> double {
     double v = 0.0;
     for (int k = 0; k < ninc; ++k) {
       const int j = slot_dyn[k * deg + p];
@@ -494,9 +506,21 @@ __global__ void __launch_bounds__(64) k_assemble_elements(tfm_mesh_t m, const do
     const int64_t row = n * NDOF + a;
     const int64_t col = (int64_t)m.adj[a0 + p] * NDOF + b;
     if (con[row] || con[col]) v = (row == col) ? 1.0 : 0.0;
+    return v;
+  };
+  for (int q = tid; q < ND2 * deg; q += 64) {
+    const int a = q / L, rem = q - a * L, p = rem / NDOF, b = rem - p * NDOF;
+    const double v = entry(a, p, b);
     if (v64) v64[vb + q] = v;
-    if (v32) v32[vb + q] = (float)v;
+    if (v32 && p != pself) v32[vb + q] = (float)v;
     if (p == pself) dblk[n * ND2 + a * NDOF + b] = v;
+  }
+  // FP32 difference form: the diagonal block holds the FP64 row sums
+  if (v32 && tid < ND2) {
+    const int a = tid / NDOF, b = tid - a * NDOF;
+    double r = 0.0;
+    for (int p = 0; p < deg; ++p) r += entry(a, p, b);
+    v32[vb + a * L + pself * NDOF + b] = (float)r;
   }
 }
 

@@ -124,8 +124,27 @@ __device__ __forceinline__ void stv(T *p, const Vec<T, K> &v) {
 }
 
 // Neighbour blocks per lane whose loads are issued before any FMA: two for
-// 16-lane teams (a 27-block Hexa1 row in one pass), one for 32-lane teams.
-template <int TL, int NRP> constexpr int hoisted_blocks() { return (TL < 32 && NRP <= 2) ? 2 : 1; }
+// 16-lane FP64 teams (a 27-block Hexa1 row in one pass), one for 32-lane
+// teams and for the FP32 products summed in FP64 (whose wider products
+// would otherwise cost the kernel two waves per SIMD).
+template <typename T, int TL, int NRP> constexpr int hoisted_blocks() {
+  return (TL < 32 && NRP <= 2 && sizeof(T) == 8) ? 2 : 1;
+}
+
+// FP32 values are stored in difference form: the diagonal block of node i
+// holds the row-sum block R_i = sum_j K_ij (summed in FP64, then rounded)
+// instead of K_ii, and the product is
+//   (K x)_i = sum_{j != i} K_ij (x_j - x_i) + R_i x_i,
+// an identity for any matrix. Rounding K_ij to FP32 then perturbs K x by
+// terms of size eps |K| |x_j - x_i| instead of eps |K| |x|: for a smooth x,
+// where K x is small by cancellation, the FP32 operator stays close to the
+// FP64 one, and the FP32 CG between two FP64 refinements loses far fewer
+// iterations to the mismatch. With one or two right-hand sides the FP32
+// products are also accumulated in FP64 (no cost on a bandwidth-bound
+// kernel); the many-rhs products keep FP32 sums for their register budget.
+template <typename T> constexpr bool diff_form() { return sizeof(T) == 4; }
+template <typename T, int NRP>
+using acc_t = typename std::conditional<(sizeof(T) == 4 && NRP <= 2), double, T>::type;
 
 // A team of TL lanes per node; lane l owns the neighbour blocks p = l, l+TL,
 // ...: it loads adj[p], the ND rows of ND values of the block, and the
@@ -149,24 +168,37 @@ __device__ __forceinline__ NodePtr node_ptr(const tfm_mesh_t &m, int64_t n) {
 template <typename T, int ND, int TL, int NRP, bool D16, bool BFLY = true>
 __device__ __forceinline__ void node_product_at(const tfm_mesh_t &m, const T *__restrict__ val,
                                                 const T *__restrict__ x, int64_t n,
-                                                const NodePtr &np, int lane, T (&acc)[ND][NRP]) {
+                                                const NodePtr &np, int lane,
+                                                acc_t<T, NRP> (&acc)[ND][NRP]) {
+  using A = acc_t<T, NRP>;
   // neighbour k of the node: from the 16-bit offsets (D16) or the int32 indices
   auto nbr = [&](int k) -> int64_t { return D16 ? n + (int64_t)m.adj16[k] : (int64_t)m.adj[k]; };
-  constexpr int KB = hoisted_blocks<TL, NRP>();
+  constexpr int KB = hoisted_blocks<T, TL, NRP>();
+  // products and sums in A: a product of two FP32 values is exact in FP64,
+  // so the row sum carries no rounding beyond that of the stored operands
+  A sum[ND][NRP];
 #pragma unroll
   for (int a = 0; a < ND; ++a)
 #pragma unroll
-    for (int r = 0; r < NRP; ++r) acc[a][r] = T(0);
+    for (int r = 0; r < NRP; ++r) sum[a][r] = A(0);
   const int a0 = np.a0;
   const int deg = np.deg;
   const T *vn = val + np.v0;
-  auto fma_block = [&](const Vec<T, ND> (&rows)[ND], const Vec<T, ND * NRP> &xv) {
+  // difference form: the node's own entries, subtracted from every neighbour's
+  Vec<T, ND * NRP> xs{};
+  if constexpr (diff_form<T>())
+    if (deg > 0) xs = ldv<T, ND * NRP>(x + n * ND * NRP);
+  auto fma_block = [&](const Vec<T, ND> (&rows)[ND], const Vec<T, ND * NRP> &xv, bool self) {
+    T d[ND * NRP];
+#pragma unroll
+    for (int q = 0; q < ND * NRP; ++q)
+      d[q] = (diff_form<T>() && !self) ? xv.v[q] - xs.v[q] : xv.v[q];
 #pragma unroll
     for (int a = 0; a < ND; ++a)
 #pragma unroll
       for (int b = 0; b < ND; ++b)
 #pragma unroll
-        for (int r = 0; r < NRP; ++r) acc[a][r] += rows[a].v[b] * xv.v[b * NRP + r];
+        for (int r = 0; r < NRP; ++r) sum[a][r] += (A)rows[a].v[b] * (A)d[b * NRP + r];
   };
   if (deg > 0) {
     const int L = ND * deg;
@@ -184,20 +216,26 @@ __device__ __forceinline__ void node_product_at(const tfm_mesh_t &m, const T *__
     }
 #pragma unroll
     for (int k = 0; k < KB; ++k)
-      if (col[k] >= 0) fma_block(rows[k], ldv<T, ND * NRP>(x + (int64_t)col[k] * ND * NRP));
+      if (col[k] >= 0)
+        fma_block(rows[k], ldv<T, ND * NRP>(x + (int64_t)col[k] * ND * NRP), col[k] == n);
     for (int p = lane + KB * TL; p < deg; p += TL) {  // rows wider than the hoisted blocks
       Vec<T, ND> rw[ND];
 #pragma unroll
       for (int a = 0; a < ND; ++a) rw[a] = ldv<T, ND>(vn + a * L + p * ND);
-      fma_block(rw, ldv<T, ND * NRP>(x + nbr(a0 + p) * ND * NRP));
+      const int64_t c = nbr(a0 + p);
+      fma_block(rw, ldv<T, ND * NRP>(x + c * ND * NRP), c == n);
     }
   } else if (lane < ND) {  // orphan node: only its diagonal is stored
 #pragma unroll
     for (int a = 0; a < ND; ++a)
       if (a == lane)
 #pragma unroll
-        for (int r = 0; r < NRP; ++r) acc[a][r] = vn[a] * x[(n * ND + a) * NRP + r];
+        for (int r = 0; r < NRP; ++r) sum[a][r] = (A)vn[a] * (A)x[(n * ND + a) * NRP + r];
   }
+#pragma unroll
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int r = 0; r < NRP; ++r) acc[a][r] = sum[a][r];
   if constexpr (BFLY) {
 #pragma unroll
     for (int off = 1; off < TL; off <<= 1)
@@ -227,7 +265,7 @@ __device__ __forceinline__ void node_product_lds(const tfm_mesh_t &m, const T *_
                                                  T (&outv)[(ND * NRP + TL - 1) / TL]) {
   constexpr int NE = ND * NRP, KE = (NE + TL - 1) / TL, PAD = NE + 1;
   __shared__ T red[BS / TL][TL][PAD];
-  T acc[ND][NRP];
+  acc_t<T, NRP> acc[ND][NRP];
   node_product_at<T, ND, TL, NRP, D16, false>(m, val, x, n, np, lane, acc);
 #pragma unroll
   for (int a = 0; a < ND; ++a)
@@ -252,7 +290,7 @@ __device__ __forceinline__ void node_product_lds(const tfm_mesh_t &m, const T *_
 template <typename T, int ND, int TL, int NRP, bool D16>
 __device__ __forceinline__ void node_product(const tfm_mesh_t &m, const T *__restrict__ val,
                                              const T *__restrict__ x, int64_t n, int lane,
-                                             T (&acc)[ND][NRP]) {
+                                             acc_t<T, NRP> (&acc)[ND][NRP]) {
   node_product_at<T, ND, TL, NRP, D16>(m, val, x, n, node_ptr(m, n), lane, acc);
 }
 
@@ -281,6 +319,11 @@ __device__ __forceinline__ void node_product_mr(const tfm_mesh_t &m, const T *__
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[a][t] = T(0);
   const T *vn = val + np.v0;
+  Vec<T, 4> xs[ND] = {};  // difference form: the node's own entries of group g
+  if constexpr (diff_form<T>())
+    if (np.deg > 0)
+#pragma unroll
+      for (int b = 0; b < ND; ++b) xs[b] = ldv<T, 4>(x + (n * ND + b) * NRP + 4 * g);
   auto block = [&](int p, int64_t c) {
     Vec<T, ND> rows[ND];
     Vec<T, 4> xv[ND];
@@ -288,6 +331,11 @@ __device__ __forceinline__ void node_product_mr(const tfm_mesh_t &m, const T *__
     for (int a = 0; a < ND; ++a) rows[a] = ldv<T, ND>(vn + a * ND * np.deg + p * ND);
 #pragma unroll
     for (int b = 0; b < ND; ++b) xv[b] = ldv<T, 4>(x + (c * ND + b) * NRP + 4 * g);
+    if (diff_form<T>() && c != n)
+#pragma unroll
+      for (int b = 0; b < ND; ++b)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) xv[b].v[t] -= xs[b].v[t];
 #pragma unroll
     for (int a = 0; a < ND; ++a)
 #pragma unroll
@@ -353,13 +401,13 @@ __global__ void __launch_bounds__(BS) k_spmv(tfm_mesh_t m, const T *__restrict__
       for (int k = 0; k < KE; ++k)
         if (lane + k * TL < ND * NRP) y[n * ND * NRP + lane + k * TL] = outv[k];
     } else {
-      T acc[ND][NRP];
+      acc_t<T, NRP> acc[ND][NRP];
       node_product<T, ND, TL, NRP, D16>(m, val, x, n, lane, acc);
 #pragma unroll
       for (int a = 0; a < ND; ++a)
 #pragma unroll
         for (int r = 0; r < NRP; ++r)
-          if ((a * NRP + r) % TL == lane) y[(n * ND + a) * NRP + r] = acc[a][r];
+          if ((a * NRP + r) % TL == lane) y[(n * ND + a) * NRP + r] = (T)acc[a][r];
     }
   }
 }
@@ -619,7 +667,8 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
         sp[k] = s[nb + e];
       }
     }
-    T acc[ND][NRP], am[ND][4], outv[KE];
+    acc_t<T, NRP> acc[ND][NRP];
+    T am[ND][4], outv[KE];
     if constexpr (rhs_lanes<TL, NRP>())
       node_product_mr<T, ND, NRP, D16>(m, val, u, n, cur, lane, am);
     else if constexpr (lds_reduce<T, TL, NRP>())
@@ -640,7 +689,7 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
           for (int a = 0; a < ND; ++a)
 #pragma unroll
             for (int r = 0; r < NRP; ++r)
-              if (a * NRP + r == e) w = acc[a][r];
+              if (a * NRP + r == e) w = (T)acc[a][r];
         }
         const T pv = up[k] + b * pp[k];
         const T sv = w + b * sp[k];
@@ -668,6 +717,15 @@ __global__ void __launch_bounds__(BS) k_pcg_spmv_dir(tfm_mesh_t m, const T *__re
 template <typename T, int ND, int TL, int NRP, bool CO, bool D16>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_pcg_spmv_dir_w8(tfm_mesh_t m, const T *__restrict__ val, const T *__restrict__ u,
+                  T *__restrict__ p, T *__restrict__ s, int it, int seg_first,
+                  const double *part_in, int n_in, PcgState st) {
+  spmv_dir_body<T, ND, TL, NRP, CO, D16>(m, val, u, p, s, it, seg_first, part_in, n_in, st);
+}
+
+// Held to 80 VGPRs (6 waves per SIMD), for kernels that spill at 64.
+template <typename T, int ND, int TL, int NRP, bool CO, bool D16>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6)))
+k_pcg_spmv_dir_w6(tfm_mesh_t m, const T *__restrict__ val, const T *__restrict__ u,
                   T *__restrict__ p, T *__restrict__ s, int it, int seg_first,
                   const double *part_in, int n_in, PcgState st) {
   spmv_dir_body<T, ND, TL, NRP, CO, D16>(m, val, u, p, s, it, seg_first, part_in, n_in, st);
@@ -1192,10 +1250,13 @@ static int spmv_dir_launch_d(hipStream_t s, const SpmvShape &sh, const tfm_mesh_
                        val, u, p, sv, it, seg_first, part_in, n_in, st);
   } else {
     // grid-stride over the nodes with every block resident in one round;
-    // the 8-wave variant unless holding it to 64 VGPRs spills registers
+    // the 8-wave variant unless holding it to 64 VGPRs spills registers,
+    // then the 6-wave one unless 80 VGPRs spill
     static const auto kern =
         pick_w8((const void *)k_pcg_spmv_dir_w8<T, ND, TL, NRP, false, D16>)
             ? k_pcg_spmv_dir_w8<T, ND, TL, NRP, false, D16>
+        : pick_w8((const void *)k_pcg_spmv_dir_w6<T, ND, TL, NRP, false, D16>)
+            ? k_pcg_spmv_dir_w6<T, ND, TL, NRP, false, D16>
             : k_pcg_spmv_dir<T, ND, TL, NRP, false, D16>;
     static const int resident = resident_grid((const void *)kern);
     *g = grid_for(m.n_nod, BS / TL, std::min(sh.cap, resident));

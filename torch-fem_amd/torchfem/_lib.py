@@ -135,6 +135,15 @@ _SIGNATURES: dict[str, tuple[list[Any], Any]] = {
         [P, P, I64, P, P, P, P, P, F64, C.POINTER(I32), C.POINTER(F64)],
         C.c_int,
     ),
+    "tfm_facet_count": ([C.POINTER(Mesh), C.POINTER(I32), C.POINTER(I32)], C.c_int),
+    "tfm_boundary_facets": (
+        [P, C.POINTER(Mesh), P, P, P, P, P, P, I64, C.POINTER(I64)],
+        C.c_int,
+    ),
+    "tfm_facet_nodes": ([P, C.POINTER(Mesh), P, I64, P], C.c_int),
+    "tfm_facet_load": ([P, C.POINTER(Mesh), P, P, P, I64, P, I32, I32, P, P], C.c_int),
+    "tfm_body_load": ([P, C.POINTER(Mesh), P, P, I32, I32, P, P, P], C.c_int),
+    "tfm_shape_gradients": ([P, C.POINTER(Mesh), P, I32, P, P, P], C.c_int),
 }
 
 

@@ -338,48 +338,53 @@ class FEM(ABC):
 
     # ---------------------------------------------------- shape functions
     def eval_shape_functions(self, xi: Tensor) -> tuple[Tensor, Tensor, Tensor]:
-        """N, B = J^{-1} dN/dxi and detJ at local points (base.py:293-314)."""
-        nodes = self.nodes[self.elements, :]
-        xi = torch.as_tensor(xi, dtype=torch.float64).to(self.device)
-        b = self.etype.B(xi).to(self.device)
-        J = torch.einsum("...iN, ANj -> ...Aij", b, nodes)
-        detJ = torch.linalg.det(J)
-        if torch.any(detJ <= 0.0):
-            raise ValueError("Negative Jacobian. Check element numbering.")
-        B = torch.einsum("...Eij,...jN->...EiN", torch.linalg.inv(J), b)
-        return self.etype.N(xi).to(self.device), B, detJ
+        """N, B = J^{-1} dN/dxi and detJ at local points xi [.., dim]
+        (base.py:293-314): B [.., n_elem, dim, npe] and detJ [.., n_elem] come
+        from one HIP kernel over (point, element) pairs; raises on detJ <= 0."""
+        xi = torch.as_tensor(xi, dtype=torch.float64)
+        lead = xi.shape[:-1]
+        pts = xi.reshape(-1, xi.shape[-1]).to(self.device).contiguous()
+        if pts.shape[1] != self.n_dim:
+            raise ValueError(f"local points need {self.n_dim} coordinates")
+        pat = self._pattern
+        nq = pts.shape[0]
+        B = torch.empty(nq, self.n_elem, self.n_dim, pat.npe, dtype=torch.float64,
+                        device=self.device)
+        detJ = torch.empty(nq, self.n_elem, dtype=torch.float64, device=self.device)
+        bad = torch.zeros(1, dtype=torch.int32, device=self.device)
+        L.check(L.lib().tfm_shape_gradients(pat.stream(), C.byref(pat.mesh), L.ptr(pts), nq,
+                                            L.ptr(B), L.ptr(detJ), L.ptr(bad)))
+        self._check_jacobian(bad)
+        N = self.etype.N(xi).to(self.device)
+        return N, B.reshape(*lead, self.n_elem, self.n_dim, pat.npe), detJ.reshape(*lead, self.n_elem)
 
     def compute_B(self) -> Tensor:
-        """Rigid-body modes (base.py:316-344)."""
-        n, d = self.n_nod, self.n_dof_per_node
+        """Rigid-body modes [n_dofs, n_modes] (base.py:316-344): the unit
+        translations, then the rotations e_k x X about the coordinate axes
+        (in 2-D the single in-plane rotation (y, -x)); one column of ones for
+        a scalar field."""
+        d = self.n_dof_per_node
         X = self.nodes
+        if d == 1:
+            return torch.ones((self.n_nod, 1), dtype=torch.float64, device=self.device)
+        eye = torch.eye(d, dtype=torch.float64, device=self.device)
+        trans = eye.expand(self.n_nod, d, d)  # [node, component, mode]
         if d == 3:
-            B = torch.zeros((3 * n, 6), dtype=torch.float64, device=self.device)
-            B[0::3, 0] = 1
-            B[1::3, 1] = 1
-            B[2::3, 2] = 1
-            B[1::3, 3] = -X[:, 2]
-            B[2::3, 3] = X[:, 1]
-            B[0::3, 4] = X[:, 2]
-            B[2::3, 4] = -X[:, 0]
-            B[0::3, 5] = -X[:, 1]
-            B[1::3, 5] = X[:, 0]
-        elif d == 2:
-            B = torch.zeros((2 * n, 3), dtype=torch.float64, device=self.device)
-            B[0::2, 0] = 1
-            B[1::2, 1] = 1
-            B[1::2, 2] = -X[:, 0]
-            B[0::2, 2] = X[:, 1]
+            rot = torch.stack([torch.linalg.cross(eye[k].expand_as(X), X, dim=-1)
+                               for k in range(3)], dim=-1)
         else:
-            B = torch.ones((n, 1), dtype=torch.float64, device=self.device)
-        return B
+            rot = torch.stack([X[:, 1], -X[:, 0]], dim=-1)[:, :, None]
+        return torch.cat([trans, rot], dim=-1).reshape(self.n_nod * d, -1)
 
     def integrate_shape_functions(self) -> Tensor:
+        """Integrals of the shape functions over each element [n_elem, npe]:
+        (w_g detJ_ge) contracted with N_gI over the Gauss points."""
         N, _, detJ = self.eval_shape_functions(self.etype.ipoints)
         w = self.etype.iweights.to(device=self.device, dtype=torch.float64)
-        return torch.einsum("i,in,ie->en", w, N, detJ)
+        return (detJ * w[:, None]).T @ N
 
     def integrate_field(self, field: Tensor | None = None) -> Tensor:
+        """Element integrals of a nodal field (its volume for None)."""
         w = self.integrate_shape_functions()
         if field is None:
             return w.sum(dim=1)
@@ -511,8 +516,15 @@ class FEM(ABC):
         columns replaced by the identity (base.py:398-426)."""
         pat = self._pattern
         k = k.to(device=self.device, dtype=torch.float64).contiguous()
-        mask = torch.zeros(self.n_dofs, dtype=torch.uint8, device=self.device)
-        mask[torch.as_tensor(con, device=self.device).long()] = 1
+        con = torch.as_tensor(con, device=self.device)
+        if con.dtype == torch.bool:
+            # a boolean mask over the DOFs, as is_constrained[con] = True accepts
+            if con.numel() != self.n_dofs:
+                raise ValueError("a boolean constraint mask needs one entry per DOF")
+            mask = con.reshape(-1).to(torch.uint8).contiguous()
+        else:
+            mask = torch.zeros(self.n_dofs, dtype=torch.uint8, device=self.device)
+            mask[con.long()] = 1
         val = torch.empty(pat.nnz, dtype=torch.float64, device=self.device)
         diag = torch.empty(pat.n_nod * pat.ndof**2, dtype=torch.float64, device=self.device)
         L.check(L.lib().tfm_assemble_elements(pat.stream(), C.byref(pat.mesh), L.ptr(k),
@@ -558,57 +570,100 @@ class FEM(ABC):
         self._K_cache = None if (value is None or value.numel() == 0) else value
 
     # -------------------------------------------------------------- loads
-    def _scatter(self, conn: Tensor, contrib: Tensor) -> Tensor:
-        f = torch.zeros(self.n_nod, contrib.shape[-1], dtype=torch.float64, device=self.device)
-        return f.index_add_(0, conn.ravel(), contrib.flatten(0, 1))
+    def _facet_set(self, mask: Tensor) -> dict:
+        """Boundary facets of a nodal selection, found on the device
+        (tfm_boundary_facets): their list position per (element, facet) slot,
+        the list of slots in the reference's order, and their count."""
+        sel = torch.as_tensor(mask, device=self.device).reshape(-1)
+        if sel.dtype != torch.bool or sel.numel() != self.n_nod:
+            raise ValueError("The mask must be a boolean tensor with one entry per node.")
+        sel = sel.to(torch.uint8).contiguous()
+        pat = self._pattern
+        nf, nfn = L.I32(), L.I32()
+        L.check(L.lib().tfm_facet_count(C.byref(pat.mesh), C.byref(nf), C.byref(nfn)))
+        slots = self.n_elem * nf.value
+        i32 = dict(dtype=torch.int32, device=self.device)
+        flag = torch.empty(max(slots, 1), dtype=torch.int8, device=self.device)
+        pos = torch.empty(max(slots, 1), **i32)
+        off = torch.empty(self.n_nod + 1, **i32)
+        bsum = torch.empty((self.n_nod + 1023) // 1024 + 1, **i32)
+        lst = torch.empty(max(slots, 1), **i32)
+        n = L.I64()
+        L.check(L.lib().tfm_boundary_facets(pat.stream(), C.byref(pat.mesh), L.ptr(sel),
+                                            L.ptr(flag), L.ptr(pos), L.ptr(off), L.ptr(bsum),
+                                            L.ptr(lst), slots, C.byref(n)))
+        return dict(mask=sel, pos=pos, list=lst, n=n.value, nfn=nfn.value)
 
     def _boundary_facets(self, mask: Tensor) -> Tensor:
-        """Boundary facets with all nodes in `mask` (base.py:452-470)."""
-        table = self.etype.facets.to(self.device)
-        facets = self.elements[:, table].reshape(-1, table.shape[1])
-        facets = facets[mask.to(self.device)[facets].all(dim=1)]
-        _, inv, count = torch.unique(facets.sort(dim=1).values, dim=0, return_inverse=True,
-                                     return_counts=True)
-        first = torch.full((len(count),), len(inv), device=self.device)
-        first.scatter_reduce_(0, inv, torch.arange(len(inv), device=self.device), reduce="amin")
-        return facets[first[count == 1]]
+        """Node ids [n_facets, nodes per facet] of the boundary facets whose
+        nodes all lie in `mask` (base.py:452-470), in the reference's order."""
+        fs = self._facet_set(mask)
+        out = torch.empty(fs["n"], fs["nfn"], dtype=torch.int64, device=self.device)
+        pat = self._pattern
+        L.check(L.lib().tfm_facet_nodes(pat.stream(), C.byref(pat.mesh), L.ptr(fs["list"]),
+                                        fs["n"], L.ptr(out)))
+        return out
 
-    def _integrate_facet_load(self, conn: Tensor, ftype, load: Tensor) -> Tensor:
-        """Consistent nodal loads on facets (base.py:472-494)."""
-        xi = ftype.ipoints.to(self.device)
-        J = torch.einsum("iaN,eNj->ieaj", ftype.B(xi).to(self.device), self.nodes[conn])
-        detJ = torch.sqrt(torch.linalg.det(J @ J.transpose(-1, -2)))
-        load = load.to(device=self.device, dtype=torch.float64)
+    @staticmethod
+    def _load_tensor(load, device) -> Tensor:
+        load = torch.as_tensor(load)
+        if load.requires_grad:
+            raise NotImplementedError(
+                "gradients with respect to distributed loads are not part of this build")
+        return load.to(device=device, dtype=torch.float64)
+
+    def _facet_load(self, mask: Tensor, load) -> Tensor:
+        """Consistent nodal loads of a load per unit facet measure on the
+        boundary facets of `mask` (base.py:472-494), by tfm_facet_load."""
+        load = self._load_tensor(load, self.device)
+        fs = self._facet_set(mask)
         if load.dim() == 0 and self.n_dof_per_node > 1:
-            if J.shape[-2] == 2:
-                n = torch.linalg.cross(J[..., 0, :], J[..., 1, :], dim=-1)
-            else:
-                n = torch.stack([J[..., 0, 1], -J[..., 0, 0]], dim=-1)
-            val = load * torch.nn.functional.normalize(n, dim=-1)
+            kind, k, buf = 0, self.n_dim, load.reshape(1)  # pressure along the outward normal
+        elif load.dim() == 2:
+            if load.shape[0] != fs["n"]:
+                raise ValueError(f"a per-facet load needs one row per facet ({fs['n']})")
+            kind, k, buf = 2, load.shape[1], load
         else:
-            per_facet = load if load.dim() == 2 else load.reshape(1, -1)
-            val = per_facet.expand(len(conn), -1).expand(len(xi), -1, -1)
-        w = ftype.iweights.to(device=self.device, dtype=torch.float64)
-        contrib = torch.einsum("i,in,ie,iek->enk", w, ftype.N(xi).to(self.device), detJ, val)
-        return self._scatter(conn, contrib)
+            buf = load.reshape(-1)
+            kind, k = 1, buf.numel()
+        buf = buf.contiguous()
+        fc = torch.empty(max(fs["n"] * fs["nfn"] * k, 1), dtype=torch.float64, device=self.device)
+        F = torch.empty(self.n_nod, k, dtype=torch.float64, device=self.device)
+        pat = self._pattern
+        L.check(L.lib().tfm_facet_load(pat.stream(), C.byref(pat.mesh), L.ptr(fs["mask"]),
+                                       L.ptr(fs["pos"]), L.ptr(fs["list"]), fs["n"], L.ptr(buf),
+                                       kind, k, L.ptr(fc), L.ptr(F)))
+        return F
 
     def integrate_body_load(self, load) -> Tensor:
-        """Consistent nodal loads from a load per unit volume (base.py:496-513)."""
-        load = torch.as_tensor(load, dtype=torch.float64, device=self.device)
-        per_elem = load if load.dim() == 2 else load.reshape(1, -1)
-        w = self.integrate_shape_functions() * self.volume_scale[:, None]
-        contrib = torch.einsum("en,ek->enk", w, per_elem.expand(self.n_elem, -1))
-        return self._scatter(self.elements, contrib)
+        """Consistent nodal loads from a load per unit volume, [k] for every
+        element or [n_elem, k] (base.py:496-513), by tfm_body_load."""
+        load = self._load_tensor(load, self.device)
+        per_elem = load.dim() == 2
+        if per_elem and load.shape[0] != self.n_elem:
+            raise ValueError("a per-element load needs one row per element")
+        buf = (load if per_elem else load.reshape(-1)).contiguous()
+        k = buf.shape[-1]
+        scale = self.volume_scale.to(torch.float64).contiguous()
+        pat = self._pattern
+        fe = torch.empty(max(self.n_elem * pat.npe * k, 1), dtype=torch.float64, device=self.device)
+        F = torch.empty(self.n_nod, k, dtype=torch.float64, device=self.device)
+        bad = torch.zeros(1, dtype=torch.int32, device=self.device)
+        L.check(L.lib().tfm_body_load(pat.stream(), C.byref(pat.mesh), L.ptr(scale), L.ptr(buf),
+                                      int(per_elem), k, L.ptr(fe), L.ptr(F), L.ptr(bad)))
+        self._check_jacobian(bad)
+        return F
 
     def integrate_surface_load(self, mask: Tensor, load) -> Tensor:
-        """Consistent nodal loads from a load per unit area (base.py:515-539)."""
+        """Consistent nodal loads from a load per unit area (base.py:515-539):
+        a float is a pressure along the outward normal, [k] or [n_facet, k] a
+        traction in global coordinates."""
         if self.etype.iso_dim != 3:
             raise NotImplementedError(
                 f"{type(self).__name__} has no surfaces to load. Use "
                 "integrate_line_load(...) or integrate_body_load(...) instead."
             )
-        return self._integrate_facet_load(self._boundary_facets(mask), self.etype.facet_type,
-                                          torch.as_tensor(load, dtype=torch.float64))
+        return self._facet_load(mask, load)
 
     def integrate_line_load(self, mask: Tensor, load) -> Tensor:
         """Consistent nodal loads from a load per unit length (base.py:541-568)."""
@@ -617,11 +672,11 @@ class FEM(ABC):
                 f"{type(self).__name__} has no edges to load. Use "
                 "integrate_surface_load(...) or integrate_body_load(...) instead."
             )
-        load = torch.as_tensor(load, dtype=torch.float64)
+        load = torch.as_tensor(load)
         if load.dim() == 0 and self.n_dim == 3:
             raise ValueError("A line in 3D has no unique normal, so a scalar load is ambiguous. "
                              "Pass a load vector instead.")
-        return self._integrate_facet_load(self._boundary_facets(mask), self.etype.facet_type, load)
+        return self._facet_load(mask, load)
 
     # --------------------------------------------------------------- solve
     def solve(

@@ -197,6 +197,21 @@ int tfm_internal_force(tfm_stream_t s, const tfm_mesh_t *m,
                        double *flux_mean, double *grad_gp, double *flux_gp,
                        double *work);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): tfm_internal_force without the work argument. The
+ * element forces are staged in a static device buffer of
+ * TFM_STATIC_WORK_DOUBLES doubles (32 MiB) that every GPU the library runs on
+ * holds until the process ends, so n_elem * npe * ndof must not exceed it
+ * (TFM_ERR_ARG otherwise). The call allocates nothing. All calls on one GPU
+ * share that buffer: calls that overlap on one GPU (from other streams or
+ * threads) race. */
+#define TFM_STATIC_WORK_DOUBLES (4 * 1024 * 1024)
+int tfm_internal_force_static(tfm_stream_t s, const tfm_mesh_t *m,
+                              const tfm_material_t *mat, const double *u,
+                              const double *flux_prev, double *f, double *grad_mean,
+                              double *flux_mean, double *grad_gp, double *flux_gp);
+
 /* F[n*ndof+a] = sum over the elements of node n (ascending) of
  * fe[e][local][a]: FEM.assemble_rhs (base.py:428-445) without float atomics. */
 int tfm_gather_nodes(tfm_stream_t s, const tfm_mesh_t *m, const double *fe, double *F);

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <thread>
 #include <vector>
@@ -248,6 +249,53 @@ extern "C" int tfm_internal_force(tfm_stream_t s, const tfm_mesh_t *m,
                                   double *work) {
   TFM_TRY(check_mesh(m));
   TFM_REQUIRE(mat && u && work, "null argument");
+  return launch_internal_force((hipStream_t)s, *m, *mat, u, flux_prev, f, grad_mean, flux_mean,
+                               grad_gp, flux_gp, work);
+}
+
+// ------------------------------------------------- allocation-free variant
+// tfm_internal_force_static stages the element forces in this file-scope
+// device array instead of a caller's work buffer. Host code never passes the
+// array's name to a kernel: each device's copy has its own address, taken
+// once per device with hipGetSymbolAddress and kept in g_scratch_ptr.
+static __device__ __align__(256) double scratch[TFM_STATIC_WORK_DOUBLES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Address of the current device's scratch array (looked up on its first use).
+int scratch_ptr(double **out) {
+  int dev = 0;
+  TFM_CHECK_HIP(hipGetDevice(&dev));
+  TFM_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    double *p = nullptr;
+    TFM_CHECK_HIP(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+    TFM_REQUIRE(p != nullptr, "no device address for the static work buffer");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TFM_OK;
+}
+}  // namespace
+
+extern "C" int tfm_internal_force_static(tfm_stream_t s, const tfm_mesh_t *m,
+                                         const tfm_material_t *mat, const double *u,
+                                         const double *flux_prev, double *f,
+                                         double *grad_mean, double *flux_mean,
+                                         double *grad_gp, double *flux_gp) {
+  TFM_TRY(check_mesh(m));
+  TFM_REQUIRE(mat && u, "null argument");
+  TFM_REQUIRE(m->n_elem * m->npe * m->ndof <= (int64_t)TFM_STATIC_WORK_DOUBLES,
+              "mesh too large for the static work buffer of tfm_internal_force_static "
+              "(n_elem * npe * ndof must not exceed TFM_STATIC_WORK_DOUBLES)");
+  double *work = nullptr;
+  TFM_TRY(scratch_ptr(&work));
+  // every element writes all of its npe * ndof entries before the gather
+  // reads them, so the buffer needs no clearing between calls
   return launch_internal_force((hipStream_t)s, *m, *mat, u, flux_prev, f, grad_mean, flux_mean,
                                grad_gp, flux_gp, work);
 }

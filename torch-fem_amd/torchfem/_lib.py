@@ -102,6 +102,10 @@ _SIGNATURES: dict[str, tuple[list[Any], Any]] = {
         [P, C.POINTER(Mesh), C.POINTER(Material), P, P, P, P, P, P, P, P],
         C.c_int,
     ),
+    "tfm_internal_force_static": (
+        [P, C.POINTER(Mesh), C.POINTER(Material), P, P, P, P, P, P, P],
+        C.c_int,
+    ),
     "tfm_gather_nodes": ([P, C.POINTER(Mesh), P, P], C.c_int),
     "tfm_sensitivity": (
         [P, C.POINTER(Mesh), C.POINTER(Material), P, P, I32, F64, P],

@@ -69,6 +69,15 @@ enum {
                          on the matrix-free FP64 residual                        */
 };
 
+/* Forms of the FP32 values (tfm_assemble): the rounded FP64 values, or the
+ * difference form whose nodal diagonal blocks hold the rounded FP64 row sums
+ * R_n = sum_j K_nj; the FP32 products use the difference form for one or two
+ * right-hand sides and the plain form for more. */
+enum {
+  TFM_FP32_PLAIN = 0,
+  TFM_FP32_DIFF = 1
+};
+
 /* Preconditioners. */
 enum {
   TFM_PC_JACOBI = 0,      /* inverse of diag(K) (cupy path, sparse.py:406-421) */
@@ -122,10 +131,14 @@ typedef struct tfm_system {
   const double *val64;       /* FP64 CSR values or NULL */
   const float *val32;        /* FP32 CSR values in difference form (see tfm_assemble) or NULL */
   const double *diag_blocks; /* [n_nod, ndof, ndof] nodal diagonal blocks of K */
+  int32_t val32_form;        /* TFM_FP32_* of val32: DIFF for a solve of one or two
+                                right-hand sides, PLAIN for more */
+  int32_t _pad;
 } tfm_system_t;
 
 typedef struct tfm_pattern tfm_pattern_t;
 typedef struct tfm_pcg tfm_pcg_t;
+typedef struct tfm_krylov tfm_krylov_t;
 
 /* ---------------------------------------------------------------- errors */
 const char *tfm_last_error(void);
@@ -145,6 +158,29 @@ int tfm_pattern_export(const tfm_pattern_t *h, int32_t *adj_ptr, int32_t *adj,
                        int64_t *val_ptr, int32_t *ne_ptr, int32_t *ne_elem,
                        int32_t *ne_loc, uint16_t *epos);
 int tfm_pattern_destroy(tfm_pattern_t *h);
+
+/* ------------------------------------------------- sparsity pattern (device) */
+/* The same pattern built on the device from device connectivity
+ * elements [n_elem, npe] (int32), in two calls (FEM.__init__'s pattern block,
+ * base.py:72-122, with integer atomics for counts and per-node sorts for
+ * order, so every array equals tfm_pattern_export's bit for bit).
+ * Count: fills ne_ptr [n_nod+1], ne_elem, ne_loc [n_elem*npe], adj_ptr
+ * [n_nod+1] and val_ptr [n_nod+1] (int64); work is int32 scratch of at least
+ * 3*n_nod + 16 + 4*(n_nod/1024 + 2) entries; info[0..3] (host) receives the
+ * adjacency length, nnz, the max degree and the max incidence. At most 192
+ * adjacent nodes per node. Synchronises the stream once. */
+int tfm_pattern_device_count(tfm_stream_t s, const int32_t *elements, int64_t n_elem,
+                             int32_t npe, int64_t n_nod, int32_t ndof, int32_t *ne_ptr,
+                             int32_t *ne_elem, int32_t *ne_loc, int32_t *adj_ptr,
+                             int64_t *val_ptr, int32_t *work, int64_t *info);
+/* Fill: adj [info[0]], epos [n_elem*npe*npe] and, when adj16 is not NULL, the
+ * 16-bit offsets adj[k] - (row node); *adj16_fits (host) tells whether all of
+ * them fit (else adj16 must not be used). */
+int tfm_pattern_device_fill(tfm_stream_t s, const int32_t *elements, int64_t n_elem,
+                            int32_t npe, int64_t n_nod, const int32_t *ne_ptr,
+                            const int32_t *ne_elem, const int32_t *adj_ptr, int32_t *adj,
+                            uint16_t *epos, int16_t *adj16, int32_t *work,
+                            int32_t *adj16_fits);
 
 /* ------------------------------------------- pattern expansion (device) */
 /* The reference's index arrays on the device: glob_idx rows/cols (int64,
@@ -167,21 +203,22 @@ int tfm_element_matrices(tfm_stream_t s, const tfm_mesh_t *m,
  * CSR values of K in FP64 and/or FP32 (either may be NULL) and the nodal
  * diagonal blocks. Replaces integrate_material's k plus
  * FEM.assemble_matrix (base.py:398-426) with no float atomics.
- * The FP32 values are the FP64 ones rounded, except in each node's diagonal
- * block, which holds the rounded FP64 row-sum block R_n = sum_j K_nj (the
- * "difference form": the FP32 products compute
- * sum_{j != n} K_nj (x_j - x_n) + R_n x_n, which equals K x exactly and
- * keeps the FP32 operator close to the FP64 one on smooth vectors). */
+ * The FP32 values are the FP64 ones rounded (fp32_form TFM_FP32_PLAIN), or
+ * with TFM_FP32_DIFF each node's diagonal block holds the rounded FP64
+ * row-sum block R_n = sum_j K_nj instead (the "difference form": the FP32
+ * products compute sum_{j != n} K_nj (x_j - x_n) + R_n x_n, which equals
+ * K x exactly and keeps the FP32 operator close to the FP64 one on smooth
+ * vectors). */
 int tfm_assemble(tfm_stream_t s, const tfm_mesh_t *m, const tfm_material_t *mat,
                  const uint8_t *con, double *val64, float *val32,
-                 double *diag_blocks, int32_t *bad_jac);
+                 double *diag_blocks, int32_t *bad_jac, int32_t fp32_form);
 
 /* Same Dirichlet epilogue and outputs as tfm_assemble, from caller-supplied
  * element matrices k [n_elem, npe*ndof, npe*ndof] (FEM.assemble_matrix,
  * base.py:398-426, as an ordered gather instead of index_add_). */
 int tfm_assemble_elements(tfm_stream_t s, const tfm_mesh_t *m, const double *k,
                           const uint8_t *con, double *val64, float *val32,
-                          double *diag_blocks);
+                          double *diag_blocks, int32_t fp32_form);
 
 /* Internal forces and fluxes for a nodal field u (FP64):
  * element forces fe[e][j][a] = sum_gp w detJ B^T (flux_prev + flux(grad u))
@@ -231,8 +268,7 @@ int tfm_sensitivity(tfm_stream_t s, const tfm_mesh_t *m, const tfm_material_t *m
 
 /* ----------------------------------------------------- sparse kernels */
 /* y = K x on the node-block CSR values, FP64 (prec 0) or FP32 (prec 1; the
- * values in the difference form that tfm_assemble writes, products summed
- * in FP64 for one or two right-hand sides). */
+ * values in the difference form, TFM_FP32_DIFF, products summed in FP64). */
 int tfm_spmv(tfm_stream_t s, const tfm_mesh_t *m, int32_t prec, const void *vals,
              const void *x, void *y);
 
@@ -275,11 +311,40 @@ int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t s, const tfm_system_t *sys,
 int tfm_pcg_timing(tfm_pcg_t *h, int32_t enable, double *spmv_ms,
                    int64_t *spmv_launches);
 
-/* Preconditioned CG for a general SPD CSR matrix (FP64, Jacobi). */
+/* Preconditioned CG for a general SPD CSR matrix (FP64, Jacobi). Like
+ * tfm_pcg_solve it fails with TFM_ERR_NOCONV as soon as an iteration finds
+ * p.Ap <= 0 (the matrix is not positive definite). */
 int tfm_csr_pcg_solve(tfm_pcg_t *h, tfm_stream_t s, int64_t n,
                       const int64_t *rowptr, const int32_t *col,
                       const double *val, const double *b, double *x, double rtol,
                       int32_t *iters, double *relres);
+
+/* ---------------------------------------- Krylov solvers for general CSR */
+/* Workspace handle of tfm_csr_minres_solve / tfm_csr_gmres_solve for systems
+ * of size n; `restart` is the GMRES restart length (basis vectors per cycle),
+ * at least 1 (the AmgXSolver-like handle convention, amgx.py:211-395). */
+int tfm_krylov_create(tfm_krylov_t **h, int64_t n, int32_t restart);
+int tfm_krylov_destroy(tfm_krylov_t *h);
+
+/* Preconditioned MINRES for a symmetric (possibly indefinite) CSR matrix,
+ * FP64, Jacobi preconditioner 1/|A_ii| (the "minres" method of sparse_solve,
+ * sparse.py:406-421 on the GPU, 493-502 on the CPU). x holds the initial
+ * guess; stops when ||b - A x||_2 <= rtol ||b||_2 (checked on the true
+ * residual; the Lanczos recurrence restarts from x if its estimate stopped
+ * early). TFM_ERR_NOCONV after max_iter iterations. */
+int tfm_csr_minres_solve(tfm_krylov_t *h, tfm_stream_t s, int64_t n,
+                         const int64_t *rowptr, const int32_t *col, const double *val,
+                         const double *b, double *x, double rtol, int32_t max_iter,
+                         int32_t *iters, double *relres);
+
+/* Restarted GMRES for any square CSR matrix, FP64, right Jacobi
+ * preconditioning, classical Gram-Schmidt with one reorthogonalisation pass:
+ * the solve behind sparse_solve's "spsolve" method (sparse.py:403-404,
+ * 490-492) and non-symmetric systems, to the true relative residual rtol. */
+int tfm_csr_gmres_solve(tfm_krylov_t *h, tfm_stream_t s, int64_t n,
+                        const int64_t *rowptr, const int32_t *col, const double *val,
+                        const double *b, double *x, double rtol, int32_t max_iter,
+                        int32_t *iters, double *relres);
 
 /* ------------------------------------------------------- distributed loads */
 /* Facets per element (*nf) and nodes per facet (*nfn) of the mesh's element

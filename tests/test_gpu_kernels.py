@@ -144,6 +144,9 @@ def test_fused_assembly_matches_oracle(name, general):
     expect = R[rows[diag_blk], cols[diag_blk] % nd]
     assert torch.allclose(v32[diag_blk].double(), expect, rtol=2.0**-23,
                           atol=1e-12 * float(v.abs().max()))
+    # a system for four or more right-hand sides keeps the plain rounding
+    s32p = _System(m, kind, p0, p1, sc, "mixed", nrhs=4)
+    assert torch.equal(s32p.val32.cpu(), v.float())
     # bitwise reproducible
     s64b = _System(m, kind, p0, p1, sc, "fp64")
     assert torch.equal(s64b.val64, s64.val64)
@@ -243,3 +246,71 @@ def test_sensitivity_matches_autograd(name, general):
         assert torch.allclose(out[:, : d**2], ref, rtol=1e-10, atol=1e-12 * ref.abs().max())
     if tk is not None:
         assert torch.allclose(out[:, -1], grads[1], rtol=1e-10, atol=1e-12 * grads[1].abs().max())
+
+
+def _pattern_meshes():
+    g = torch.Generator().manual_seed(3)
+    out = {}
+    nodes, el = O.rect_quad(7, 5)
+    out["Quad1"] = (nodes, el, 2)
+    nodes, el = O.cube_tetra6(4, jitter=0.1)
+    out["Tetra1-heat"] = (nodes, el, 1)
+    nodes, el = O.cube_hexa(6, 5, 4)
+    # elements in random order: the atomics fill each node's incidence list
+    # in an arbitrary order, the sort must restore the ascending one
+    out["Hexa1-shuffled"] = (nodes, el[torch.randperm(len(el), generator=g)], 3)
+    nodes, el = O.linear_to_quadratic(*O.cube_hexa(4, 4, 3))
+    out["Hexa2"] = (nodes, el, 3)
+    # two nodes that belong to no element keep only their diagonal
+    nodes, el = O.cube_hexa(3, 3, 3)
+    nodes = torch.cat([nodes, torch.tensor([[5.0, 5.0, 5.0], [6.0, 6.0, 6.0]], dtype=DT)])
+    out["Hexa1-orphans"] = (nodes, el, 3)
+    return out
+
+
+@pytest.mark.parametrize("name", list(_pattern_meshes()))
+def test_device_pattern_build_equals_the_host_builder(name):
+    """tfm_pattern_device_count/fill reproduce tfm_pattern_create's arrays bit
+    for bit (base.py:72-122 restated on the device)."""
+    from torchfem._pattern import build_device_pattern, build_host_pattern
+
+    nodes, el, ndof = _pattern_meshes()[name]
+    host = build_host_pattern(el, len(nodes), ndof)
+    dev = build_device_pattern(el.to(torch.int32).cuda().contiguous(), len(nodes), ndof,
+                               torch.device("cuda", 0))
+    n_adj = int(host["adj_ptr"][-1])
+    for key in ("ne_ptr", "ne_elem", "ne_loc", "adj_ptr", "val_ptr"):
+        assert np.array_equal(dev[key].cpu().numpy(), host[key]), key
+    assert np.array_equal(dev["adj"].cpu().numpy()[:n_adj], host["adj"])
+    assert np.array_equal(dev["epos"].cpu().numpy().view(np.uint16), host["epos"])
+    for key in ("nnz", "max_deg", "max_inc"):
+        assert dev[key] == host[key], key
+    rows = np.repeat(np.arange(len(nodes)), np.diff(host["adj_ptr"]))
+    assert dev["adj16"] is not None
+    assert np.array_equal(dev["adj16"].cpu().numpy()[:n_adj], (host["adj"] - rows).astype(np.int16))
+
+
+def test_device_pattern_rejects_bad_connectivity():
+    from torchfem._pattern import build_device_pattern
+
+    el = torch.tensor([[0, 1, 2, 7]], dtype=torch.int32).cuda()
+    with pytest.raises(L.HIPError, match="outside"):
+        build_device_pattern(el, 4, 1, torch.device("cuda", 0))
+
+
+def test_sixteen_bit_neighbour_offsets():
+    """The optional adj16 array holds adj[k] - row node exactly when every
+    offset fits in 16 bits, and is absent otherwise."""
+    from torchfem._pattern import DevicePattern
+
+    dev = torch.device("cuda", 0)
+    nodes, el = O.cube_hexa(6, 6, 6)
+    dp = DevicePattern(nodes, el, 3, "Hexa1", dev)
+    assert dp.adj16 is not None and dp.mesh.adj16
+    rows = torch.repeat_interleave(torch.arange(dp.n_nod, device=dev), torch.diff(dp.adj_ptr.long()))
+    assert torch.equal(rows + dp.adj16.long()[: len(rows)], dp.adj.long()[: len(rows)])
+    # two elements whose nodes are 40,000 apart: offsets beyond int16
+    far = torch.tensor([[0, 1, 2, 3, 40000, 40001, 40002, 40003]])
+    pts = torch.zeros(40004, 3, dtype=torch.float64)
+    dp2 = DevicePattern(pts, far, 3, "Hexa1", dev)
+    assert dp2.adj16 is None and not dp2.mesh.adj16

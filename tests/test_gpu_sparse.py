@@ -93,12 +93,13 @@ def _dense_grads(A_dense, b, w):
     return x, gA, gb
 
 
-@pytest.mark.parametrize("build,method", [(_spd, None), (_spd, "cg"), (_nonsymmetric, None)])
+@pytest.mark.parametrize("build,method", [(_spd, None), (_spd, "cg"), (_spd, "minres"),
+                                          (_nonsymmetric, None), (_nonsymmetric, "spsolve")])
 def test_adjoint_gradients_match_dense(build, method):
     A_dense, b, w = build(N, 0), torch.randn(N, dtype=torch.float64), torch.randn(N, dtype=torch.float64)
     x, gA, gb = _sparse_grads(A_dense, b, w, method)
     xd, gAd, gbd = _dense_grads(A_dense, b, w)
-    tol = 1e-10 if method is None else 1e-8
+    tol = 1e-10 if method in (None, "spsolve") else 1e-8
     assert torch.allclose(x, xd, atol=tol)
     assert torch.allclose(gb, gbd, atol=tol)
     assert torch.allclose(gA, gAd, atol=tol)
@@ -129,7 +130,7 @@ def test_solvers_do_not_share_a_default_cache():
 def test_resolve_and_describe():
     assert resolve_method(10, "cuda", "cg") == "cg"
     assert resolve_method(9999, "cuda", None) == "spsolve"
-    assert resolve_method(10000, "cuda", None) == "cg"
+    assert resolve_method(10000, "cuda", None) == "minres"
     assert "jacobi" in describe_method(20000, "cuda", None)
 
 
@@ -184,3 +185,99 @@ def test_newton_raises_without_convergence():
     with pytest.raises(RuntimeError, match="did not converge"):
         newton_solve(_residual_fn(A, f, p), u0, None, 2, 1e-12, 1e-14, 1e-12, None,
                      None, None, None, False, None, None, None, None, p)
+
+
+# ------------------------------------------------ systems of 10^4 unknowns
+def _fem_matrix(n=15):
+    """The oracle's assembled Hexa1 stiffness of an n^3 cube clamped at z=0
+    (Dirichlet rows/columns replaced by the identity): 3 (n+1)^3 unknowns,
+    symmetric positive definite."""
+    import oracle as O
+
+    nodes, el = O.cube_hexa(n + 1, n + 1, n + 1)
+    o = O.OracleModel(nodes, el, O.Hexa1, "solid", O.iso3d_C(torch.full((len(el),), 1000.0), 0.3))
+    con = torch.zeros(len(nodes), 3, dtype=torch.bool)
+    con[nodes[:, 2] == 0.0] = True
+    cidx = torch.nonzero(con.ravel()).ravel()
+    return o.csr(o.assemble_matrix(o.k0(), cidx)).tocsr(), con.ravel()
+
+
+def _coo(K):
+    import numpy as np
+
+    Kc = K.tocoo()
+    idx = torch.as_tensor(np.vstack([Kc.row, Kc.col]).astype("int64"))
+    return torch.sparse_coo_tensor(idx, torch.as_tensor(Kc.data), K.shape).coalesce().cuda()
+
+
+def _scipy(K, b):
+    import scipy.sparse.linalg as spla
+
+    return torch.as_tensor(spla.spsolve(K.tocsc(), b.numpy()))
+
+
+@pytest.mark.parametrize("method", [None, "cg", "minres", "spsolve"])
+def test_fem_matrix_of_12k_unknowns_matches_scipy(method):
+    """An assembled stiffness of 12,288 unknowns (above the 10k switch, so
+    method None is MINRES) against SciPy's direct solve."""
+    K, con = _fem_matrix()
+    assert K.shape[0] == 12288
+    b = torch.randn(K.shape[0], generator=torch.Generator().manual_seed(3), dtype=torch.float64)
+    b[con] = 0.0
+    x, M = sparse_solve(_coo(K), b.cuda(), stol=1e-12, method=method)
+    ref = _scipy(K, b)
+    assert float((x.cpu() - ref).norm() / ref.norm()) < 1e-7
+    assert (M is None) == (method == "spsolve")
+
+
+def _nonsym_matrix(K, seed=0):
+    """K plus a skew part on K's own pattern (a convection-like term): no
+    longer symmetric, still invertible (its symmetric part is K)."""
+    import scipy.sparse as sp
+
+    Kc = K.tocoo()
+    g = torch.Generator().manual_seed(seed)
+    up = Kc.row < Kc.col
+    r, c = Kc.row[up], Kc.col[up]
+    v = (0.2 * torch.rand(len(r), generator=g, dtype=torch.float64) * abs(K.diagonal()).mean()).numpy()
+    S = sp.coo_matrix((v, (r, c)), shape=K.shape)
+    return (K + S - S.T).tocsr()
+
+
+@pytest.mark.parametrize("method", [None, "spsolve"])
+def test_non_symmetric_system_and_its_transposed_adjoint(method):
+    """A non-symmetric matrix of 12k unknowns: the solve (GMRES) and the
+    adjoint A^T lam = w of differentiable_sparse_solve against SciPy; method
+    None sends it to GMRES instead of MINRES, and 'minres' refuses it."""
+    K, con = _fem_matrix()
+    A = _nonsym_matrix(K)
+    Ac = _coo(A)
+    vals = Ac._values().clone().requires_grad_(True)
+    Ag = torch.sparse_coo_tensor(Ac._indices(), vals, A.shape).coalesce()
+    b = torch.randn(A.shape[0], generator=torch.Generator().manual_seed(4), dtype=torch.float64)
+    w = torch.randn(A.shape[0], generator=torch.Generator().manual_seed(5), dtype=torch.float64)
+    bg = b.cuda().requires_grad_(True)
+    x = differentiable_sparse_solve(Ag, bg, stol=1e-12, method=method)
+    assert float((x.detach().cpu() - _scipy(A, b)).norm() / _scipy(A, b).norm()) < 1e-8
+    gv, gb = torch.autograd.grad((x * w.cuda()).sum(), [vals, bg])
+    lam = _scipy(A.T.tocsr(), w)
+    assert float((gb.cpu() - lam).norm() / lam.norm()) < 1e-8
+    with pytest.raises(ValueError, match="symmetric"):
+        sparse_solve(Ac, b.cuda(), method="minres")
+
+
+def test_indefinite_matrix_minres_converges_and_cg_fails_fast():
+    """K - sigma I with sigma inside the spectrum is symmetric indefinite:
+    MINRES solves it, CG stops at the first p.Ap <= 0 with an error instead
+    of running to its iteration limit."""
+    import scipy.sparse as sp
+
+    K, _ = _fem_matrix(n=10)
+    sigma = 0.05 * float(abs(K.diagonal()).mean())
+    A = (K - sigma * sp.eye(K.shape[0])).tocsr()
+    b = torch.randn(A.shape[0], generator=torch.Generator().manual_seed(6), dtype=torch.float64)
+    x, _ = sparse_solve(_coo(A), b.cuda(), stol=1e-10, method="minres")
+    ref = _scipy(A, b)
+    assert float((x.cpu() - ref).norm() / ref.norm()) < 1e-6
+    with pytest.raises(RuntimeError, match="positive definite"):
+        sparse_solve(_coo(A), b.cuda(), stol=1e-10, method="cg")

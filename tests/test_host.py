@@ -110,20 +110,3 @@ def test_hexa_cube_64_pattern_size_closed_form():
     hp = build_host_pattern(el, len(nodes), 3)
     assert hp["nnz"] == O.hexa_nnz_closed_form(65) == 64_701_513
     assert hp["max_deg"] == 27 and hp["max_inc"] == 8
-
-
-def test_sixteen_bit_neighbour_offsets():
-    """The optional adj16 array holds adj[k] - row node exactly when every
-    offset fits in 16 bits, and is absent otherwise."""
-    from torchfem._pattern import DevicePattern
-
-    nodes, el = O.cube_hexa(6, 6, 6)
-    dp = DevicePattern(nodes, el, 3, "Hexa1", torch.device("cpu"))
-    assert dp.adj16 is not None and dp.mesh.adj16
-    rows = torch.repeat_interleave(torch.arange(dp.n_nod), torch.diff(dp.adj_ptr.long()))
-    assert torch.equal(rows + dp.adj16.long(), dp.adj.long())
-    # two elements whose nodes are 40,000 apart: offsets beyond int16
-    far = torch.tensor([[0, 1, 2, 3, 40000, 40001, 40002, 40003]])
-    pts = torch.zeros(40004, 3, dtype=torch.float64)
-    dp2 = DevicePattern(pts, far, 3, "Hexa1", torch.device("cpu"))
-    assert dp2.adj16 is None and not dp2.mesh.adj16

@@ -17,7 +17,7 @@ CSRC = HERE / "csrc"
 INCLUDE = HERE.parent / "include"
 OUT = HERE / "torchfem" / "libtfm_hip.so"
 BUILD_DIR = HERE / "_build"
-SOURCES = ["assembly.hip", "solver.hip", "capi.hip", "loads.hip"]
+SOURCES = ["assembly.hip", "solver.hip", "capi.hip", "loads.hip", "krylov.hip", "pattern.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -186,7 +186,8 @@ template <class E> constexpr int assemble_chunk() { return E::NPE >= 16 ? 4 : 64
 template <class E, int NDOF, int MAT>
 __global__ void __launch_bounds__(64) k_assemble_rows(tfm_mesh_t m, tfm_material_t mat,
                                                       const uint8_t *con, double *v64,
-                                                      float *v32, double *dblk, int32_t *bad) {
+                                                      float *v32, double *dblk, int32_t *bad,
+                                                      int diff32) {
   constexpr int NPE = E::NPE, NGP = E::NGP, D = E::DIM, NP = mat_nparam(MAT, D);
   constexpr int ND2 = NDOF * NDOF, KCH = assemble_chunk<E>();
   __shared__ double X[KCH][NPE][D];
@@ -295,12 +296,12 @@ __global__ void __launch_bounds__(64) k_assemble_rows(tfm_mesh_t m, tfm_material
     if (con[row] || con[col]) v = (row == col) ? 1.0 : 0.0;
     acc[q] = v;
     if (v64) v64[vb + q] = v;
-    if (v32 && p != pself) v32[vb + q] = (float)v;
+    if (v32 && (p != pself || !diff32)) v32[vb + q] = (float)v;
     if (p == pself) dblk[n * ND2 + a * NDOF + b] = v;
   }
   // FP32 difference form: the diagonal block holds the row sums of the
   // final FP64 values, added in ascending slot order
-  if (v32) {
+  if (v32 && diff32) {
     __syncthreads();
     if (tid < ND2) {
       const int a = tid / NDOF, b = tid - a * NDOF;
@@ -458,7 +459,7 @@ __global__ void k_gather_nodes(tfm_mesh_t m, const double *fe, double *F) {
 template <int NDOF>
 __global__ void __launch_bounds__(64) k_assemble_elements(tfm_mesh_t m, const double *kin,
                                                           const uint8_t *con, double *v64,
-                                                          float *v32, double *dblk) {
+                                                          float *v32, double *dblk, int diff32) {
   constexpr int ND2 = NDOF * NDOF;
   extern __shared__ signed char slot_dyn[];  // [max_inc * max_deg]
   const int64_t n = blockIdx.x;
@@ -512,11 +513,11 @@ __global__ void __launch_bounds__(64) k_assemble_elements(tfm_mesh_t m, const do
     const int a = q / L, rem = q - a * L, p = rem / NDOF, b = rem - p * NDOF;
     const double v = entry(a, p, b);
     if (v64) v64[vb + q] = v;
-    if (v32 && p != pself) v32[vb + q] = (float)v;
+    if (v32 && (p != pself || !diff32)) v32[vb + q] = (float)v;
     if (p == pself) dblk[n * ND2 + a * NDOF + b] = v;
   }
   // FP32 difference form: the diagonal block holds the FP64 row sums
-  if (v32 && tid < ND2) {
+  if (v32 && diff32 && tid < ND2) {
     const int a = tid / NDOF, b = tid - a * NDOF;
     double r = 0.0;
     for (int p = 0; p < deg; ++p) r += entry(a, p, b);
@@ -681,14 +682,15 @@ int launch_element_matrices(hipStream_t s, const tfm_mesh_t &m, const tfm_materi
 }
 
 int launch_assemble(hipStream_t s, const tfm_mesh_t &m, const tfm_material_t &mat,
-                    const uint8_t *con, double *v64, float *v32, double *dblk, int32_t *bad) {
+                    const uint8_t *con, double *v64, float *v32, double *dblk, int32_t *bad,
+                    bool diff32) {
   if (m.n_nod == 0) return TFM_OK;
   TFM_DISPATCH_ALL(m, mat, {
     constexpr int KCH = assemble_chunk<ET>();
     const size_t shm = sizeof(double) * ND * ND * (size_t)m.max_deg + (size_t)KCH * m.max_deg;
     TFM_REQUIRE(shm <= 48 * 1024, "node degree too large for the assembly kernel");
     hipLaunchKernelGGL((k_assemble_rows<ET, ND, MK>), dim3((unsigned)m.n_nod), dim3(64), shm, s,
-                       m, mat, con, v64, v32, dblk, bad);
+                       m, mat, con, v64, v32, dblk, bad, (int)diff32);
     TFM_CHECK_LAUNCH();
   });
   return TFM_OK;
@@ -717,14 +719,15 @@ int launch_internal_force(hipStream_t s, const tfm_mesh_t &m, const tfm_material
 }
 
 int launch_assemble_elements(hipStream_t s, const tfm_mesh_t &m, const double *k,
-                             const uint8_t *con, double *v64, float *v32, double *dblk) {
+                             const uint8_t *con, double *v64, float *v32, double *dblk,
+                             bool diff32) {
   if (m.n_nod == 0) return TFM_OK;
   const size_t shm = (size_t)m.max_inc * m.max_deg;
   TFM_REQUIRE(shm <= 48 * 1024, "node degree too large for the assembly kernel");
   switch (m.ndof) {
-    case 1: hipLaunchKernelGGL(k_assemble_elements<1>, dim3((unsigned)m.n_nod), dim3(64), shm, s, m, k, con, v64, v32, dblk); break;
-    case 2: hipLaunchKernelGGL(k_assemble_elements<2>, dim3((unsigned)m.n_nod), dim3(64), shm, s, m, k, con, v64, v32, dblk); break;
-    case 3: hipLaunchKernelGGL(k_assemble_elements<3>, dim3((unsigned)m.n_nod), dim3(64), shm, s, m, k, con, v64, v32, dblk); break;
+    case 1: hipLaunchKernelGGL(k_assemble_elements<1>, dim3((unsigned)m.n_nod), dim3(64), shm, s, m, k, con, v64, v32, dblk, (int)diff32); break;
+    case 2: hipLaunchKernelGGL(k_assemble_elements<2>, dim3((unsigned)m.n_nod), dim3(64), shm, s, m, k, con, v64, v32, dblk, (int)diff32); break;
+    case 3: hipLaunchKernelGGL(k_assemble_elements<3>, dim3((unsigned)m.n_nod), dim3(64), shm, s, m, k, con, v64, v32, dblk, (int)diff32); break;
     default: TFM_REQUIRE(false, "ndof must be 1, 2 or 3");
   }
   TFM_CHECK_LAUNCH();

@@ -226,20 +226,24 @@ extern "C" int tfm_element_matrices(tfm_stream_t s, const tfm_mesh_t *m,
 
 extern "C" int tfm_assemble(tfm_stream_t s, const tfm_mesh_t *m, const tfm_material_t *mat,
                             const uint8_t *con, double *val64, float *val32,
-                            double *diag_blocks, int32_t *bad_jac) {
+                            double *diag_blocks, int32_t *bad_jac, int32_t fp32_form) {
   TFM_TRY(check_mesh(m));
   TFM_REQUIRE(mat && con && diag_blocks && bad_jac, "null argument");
   TFM_REQUIRE(val64 || val32, "no output values requested");
-  return launch_assemble((hipStream_t)s, *m, *mat, con, val64, val32, diag_blocks, bad_jac);
+  TFM_REQUIRE(fp32_form == TFM_FP32_PLAIN || fp32_form == TFM_FP32_DIFF, "unknown FP32 form");
+  return launch_assemble((hipStream_t)s, *m, *mat, con, val64, val32, diag_blocks, bad_jac,
+                         fp32_form == TFM_FP32_DIFF);
 }
 
 extern "C" int tfm_assemble_elements(tfm_stream_t s, const tfm_mesh_t *m, const double *k,
                                      const uint8_t *con, double *val64, float *val32,
-                                     double *diag_blocks) {
+                                     double *diag_blocks, int32_t fp32_form) {
   TFM_TRY(check_mesh(m));
   TFM_REQUIRE(k && con && diag_blocks, "null argument");
   TFM_REQUIRE(val64 || val32, "no output values requested");
-  return launch_assemble_elements((hipStream_t)s, *m, k, con, val64, val32, diag_blocks);
+  TFM_REQUIRE(fp32_form == TFM_FP32_PLAIN || fp32_form == TFM_FP32_DIFF, "unknown FP32 form");
+  return launch_assemble_elements((hipStream_t)s, *m, k, con, val64, val32, diag_blocks,
+                                  fp32_form == TFM_FP32_DIFF);
 }
 
 extern "C" int tfm_internal_force(tfm_stream_t s, const tfm_mesh_t *m,

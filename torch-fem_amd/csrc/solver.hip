@@ -36,6 +36,7 @@ constexpr int RBV = 512;  // max blocks of the vector kernels whose partials eve
 constexpr int RBW = 4096; // max blocks of the vector kernels when a scalar kernel sums them
 constexpr int MAXR = 16;  // right-hand sides per solve
 constexpr int NSCAL = 4;  // per-iteration scalars: rz, pq, rr, spare
+constexpr int PCG_CHK_MAX = 64;  // most iterations between two host checks
 enum { S_RZ = 0, S_PQ = 1, S_RR = 2 };
 
 // ---------------------------------------------------------- reductions
@@ -131,18 +132,21 @@ template <typename T, int TL, int NRP> constexpr int hoisted_blocks() {
   return (TL < 32 && NRP <= 2 && sizeof(T) == 8) ? 2 : 1;
 }
 
-// FP32 values are stored in difference form: the diagonal block of node i
-// holds the row-sum block R_i = sum_j K_ij (summed in FP64, then rounded)
-// instead of K_ii, and the product is
+// FP32 values for one or two right-hand sides are stored in difference
+// form: the diagonal block of node i holds the row-sum block
+// R_i = sum_j K_ij (summed in FP64, then rounded) instead of K_ii, and the
+// product is
 //   (K x)_i = sum_{j != i} K_ij (x_j - x_i) + R_i x_i,
 // an identity for any matrix. Rounding K_ij to FP32 then perturbs K x by
 // terms of size eps |K| |x_j - x_i| instead of eps |K| |x|: for a smooth x,
 // where K x is small by cancellation, the FP32 operator stays close to the
 // FP64 one, and the FP32 CG between two FP64 refinements loses far fewer
-// iterations to the mismatch. With one or two right-hand sides the FP32
-// products are also accumulated in FP64 (no cost on a bandwidth-bound
-// kernel); the many-rhs products keep FP32 sums for their register budget.
-template <typename T> constexpr bool diff_form() { return sizeof(T) == 4; }
+// iterations to the mismatch. These products are also summed in FP64 (a
+// product of two FP32 values is exact in FP64; no cost on a bandwidth-bound
+// kernel). The many-rhs kernels keep the plain rounded values and FP32 sums:
+// their registers hold ND*NRP sums already, and the L2-bound gathers, not
+// the iteration count, limit them.
+template <typename T, int NRP> constexpr bool diff_form() { return sizeof(T) == 4 && NRP <= 2; }
 template <typename T, int NRP>
 using acc_t = typename std::conditional<(sizeof(T) == 4 && NRP <= 2), double, T>::type;
 
@@ -185,14 +189,16 @@ __device__ __forceinline__ void node_product_at(const tfm_mesh_t &m, const T *__
   const int deg = np.deg;
   const T *vn = val + np.v0;
   // difference form: the node's own entries, subtracted from every neighbour's
-  Vec<T, ND * NRP> xs{};
-  if constexpr (diff_form<T>())
+  Vec<T, diff_form<T, NRP>() ? ND * NRP : 1> xs{};
+  if constexpr (diff_form<T, NRP>())
     if (deg > 0) xs = ldv<T, ND * NRP>(x + n * ND * NRP);
   auto fma_block = [&](const Vec<T, ND> (&rows)[ND], const Vec<T, ND * NRP> &xv, bool self) {
     T d[ND * NRP];
 #pragma unroll
-    for (int q = 0; q < ND * NRP; ++q)
-      d[q] = (diff_form<T>() && !self) ? xv.v[q] - xs.v[q] : xv.v[q];
+    for (int q = 0; q < ND * NRP; ++q) {
+      if constexpr (diff_form<T, NRP>()) d[q] = self ? xv.v[q] : xv.v[q] - xs.v[q];
+      else d[q] = xv.v[q];
+    }
 #pragma unroll
     for (int a = 0; a < ND; ++a)
 #pragma unroll
@@ -319,11 +325,6 @@ __device__ __forceinline__ void node_product_mr(const tfm_mesh_t &m, const T *__
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[a][t] = T(0);
   const T *vn = val + np.v0;
-  Vec<T, 4> xs[ND] = {};  // difference form: the node's own entries of group g
-  if constexpr (diff_form<T>())
-    if (np.deg > 0)
-#pragma unroll
-      for (int b = 0; b < ND; ++b) xs[b] = ldv<T, 4>(x + (n * ND + b) * NRP + 4 * g);
   auto block = [&](int p, int64_t c) {
     Vec<T, ND> rows[ND];
     Vec<T, 4> xv[ND];
@@ -331,11 +332,6 @@ __device__ __forceinline__ void node_product_mr(const tfm_mesh_t &m, const T *__
     for (int a = 0; a < ND; ++a) rows[a] = ldv<T, ND>(vn + a * ND * np.deg + p * ND);
 #pragma unroll
     for (int b = 0; b < ND; ++b) xv[b] = ldv<T, 4>(x + (c * ND + b) * NRP + 4 * g);
-    if (diff_form<T>() && c != n)
-#pragma unroll
-      for (int b = 0; b < ND; ++b)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) xv[b].v[t] -= xs[b].v[t];
 #pragma unroll
     for (int a = 0; a < ND; ++a)
 #pragma unroll
@@ -1326,7 +1322,7 @@ struct tfm_pcg {
   int res_cap = 0;                                         // rows (pairs) of part_res
   int *h_done = nullptr;                                   // pinned host mirrors
   double *h_res = nullptr;
-  double *h_scal = nullptr;                                // one iteration's scalars
+  double *h_scal = nullptr;                                // a batch's scalars
   // optional event timing of the SpMV launches (kernel A) of the iterations
   int timing = 0;  // 0: off; k: event pair around the SpMV of every k-th iteration
   double spmv_ms = 0.0;
@@ -1366,7 +1362,8 @@ static int pcg_alloc(tfm_pcg *h) {
   TFM_CHECK_HIP(hipMalloc(&h->alldone, sizeof(int)));
   TFM_CHECK_HIP(hipHostMalloc(&h->h_done, sizeof(int) * (nrp + 1), hipHostMallocDefault));
   TFM_CHECK_HIP(hipHostMalloc(&h->h_res, sizeof(double) * 4, hipHostMallocDefault));
-  TFM_CHECK_HIP(hipHostMalloc(&h->h_scal, sizeof(double) * nrp * NSCAL, hipHostMallocDefault));
+  TFM_CHECK_HIP(hipHostMalloc(&h->h_scal, sizeof(double) * PCG_CHK_MAX * nrp * NSCAL,
+                              hipHostMallocDefault));
   for (auto &e : h->ev) TFM_CHECK_HIP(hipEventCreate(&e));
   return TFM_OK;
 }
@@ -1542,7 +1539,7 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
   // Batches between host checks: the first of a segment has CHK iterations;
   // later ones are sized from the contraction of rr observed so far, to end
   // the segment with few empty launches and few host round trips.
-  const int CHK = 12, CHK_MIN = 4, CHK_MAX = 64;
+  const int CHK = 12, CHK_MIN = 4, CHK_MAX = PCG_CHK_MAX;
   int it = 0, n_outer = 0, seg_first = 0;
   bool any_live = mask != 0;
   while (any_live) {
@@ -1580,8 +1577,9 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
         TFM_CHECK_LAUNCH();
       }
       TFM_CHECK_HIP(hipMemcpyAsync(h->h_done, h->done, sizeof(int) * NRP, hipMemcpyDeviceToHost, s));
-      TFM_CHECK_HIP(hipMemcpyAsync(h->h_scal, h->scal + (size_t)(stop - 1) * NRP * NSCAL,
-                                   sizeof(double) * NRP * NSCAL, hipMemcpyDeviceToHost, s));
+      TFM_CHECK_HIP(hipMemcpyAsync(h->h_scal, h->scal + (size_t)start * NRP * NSCAL,
+                                   sizeof(double) * (stop - start) * NRP * NSCAL,
+                                   hipMemcpyDeviceToHost, s));
       TFM_CHECK_HIP(hipStreamSynchronize(s));
       seg_done = true;
       int end = 0;
@@ -1589,12 +1587,23 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
         seg_done &= h->h_done[r] >= 0;
         end = std::max(end, h->h_done[r]);
       }
+      // p.Kp <= 0 in an iteration that ran: K is not positive definite
+      for (int i = start; i < stop; ++i)
+        for (int r = 0; r < nrhs; ++r) {
+          if (h->h_done[r] >= 0 && i >= h->h_done[r]) continue;
+          const double eta = h->h_scal[((size_t)(i - start) * NRP + r) * NSCAL + S_PQ];
+          if (!(eta > 0.0)) {
+            set_error("the matrix is not positive definite (p.Kp <= 0 in CG)");
+            return TFM_ERR_NOCONV;
+          }
+        }
+      const double *last = h->h_scal + (size_t)(stop - 1 - start) * NRP * NSCAL;
       // next batch: iterations until the slowest running rhs reaches its
       // segment target at its observed rate (rr of iteration stop-1)
       int want = 0;
       for (int r = 0; r < nrhs; ++r) {
         if (h->h_done[r] >= 0) continue;
-        const double rr = h->h_scal[r * NSCAL + S_RR];
+        const double rr = last[r * NSCAL + S_RR];
         int k = CHK;
         if (it_seen[r] >= 0 && rr > 0.0 && rr < rr_seen[r] && htol[r] > 0.0) {
           const double rate = std::log(rr / rr_seen[r]) / (stop - 1 - it_seen[r]);
@@ -1731,6 +1740,10 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
   TFM_REQUIRE(sys->con && sys->diag_blocks, "system needs con and diag_blocks");
   TFM_REQUIRE(prec == TFM_PREC_FP64 ? sys->val64 != nullptr : sys->val32 != nullptr,
               "matrix values for the requested precision are missing");
+  TFM_REQUIRE(prec == TFM_PREC_FP64 ||
+                  sys->val32_form == (pow2_cap(nrhs) <= 2 ? TFM_FP32_DIFF : TFM_FP32_PLAIN),
+              "FP32 values in the wrong form for this number of right-hand sides "
+              "(TFM_FP32_DIFF for one or two, TFM_FP32_PLAIN for more)");
   TFM_REQUIRE(rtol > 0.0, "rtol must be positive");
   TFM_REQUIRE(inner_rtol > 0.0 && inner_rtol < 1.0, "inner_rtol must be in (0, 1)");
   TFM_REQUIRE(final_rtol > 0.0 && final_rtol <= rtol, "final_rtol must be in (0, rtol]");

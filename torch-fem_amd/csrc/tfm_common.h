@@ -267,13 +267,15 @@ __device__ __forceinline__ void flux_from_grad(const double *H, const double *mp
 int launch_element_matrices(hipStream_t s, const tfm_mesh_t &m, const tfm_material_t &mat,
                             double *k, int32_t *bad);
 int launch_assemble(hipStream_t s, const tfm_mesh_t &m, const tfm_material_t &mat,
-                    const uint8_t *con, double *v64, float *v32, double *dblk, int32_t *bad);
+                    const uint8_t *con, double *v64, float *v32, double *dblk, int32_t *bad,
+                    bool diff32);
 int launch_internal_force(hipStream_t s, const tfm_mesh_t &m, const tfm_material_t &mat,
                           const double *u, const double *flux_prev, double *f,
                           double *grad_mean, double *flux_mean, double *grad_gp,
                           double *flux_gp, double *work);
 int launch_assemble_elements(hipStream_t s, const tfm_mesh_t &m, const double *k,
-                             const uint8_t *con, double *v64, float *v32, double *dblk);
+                             const uint8_t *con, double *v64, float *v32, double *dblk,
+                             bool diff32);
 int launch_gather(hipStream_t s, const tfm_mesh_t &m, const double *fe, double *F);
 int launch_sensitivity(hipStream_t s, const tfm_mesh_t &m, const tfm_material_t &mat,
                        const double *u, const double *lam, int nrhs, double sign, double *out);

@@ -29,6 +29,7 @@ ETYPE = {"Quad1": 1, "Tetra1": 2, "Hexa1": 3, "Hexa2": 4}
 MAT_ISO, MAT_GENERAL, MAT_HEAT_ISO, MAT_HEAT_GENERAL = 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
 PC_JACOBI, PC_BLOCK_JACOBI = 0, 1
+FP32_PLAIN, FP32_DIFF = 0, 1
 
 P = C.c_void_p
 I32 = C.c_int32
@@ -78,6 +79,8 @@ class System(C.Structure):
         ("val64", P),
         ("val32", P),
         ("diag_blocks", P),
+        ("val32_form", I32),
+        ("_pad", I32),
     ]
 
 
@@ -91,13 +94,21 @@ _SIGNATURES: dict[str, tuple[list[Any], Any]] = {
     ),
     "tfm_pattern_export": ([P, P, P, P, P, P, P, P], C.c_int),
     "tfm_pattern_destroy": ([P], C.c_int),
+    "tfm_pattern_device_count": (
+        [P, P, I64, I32, I64, I32, P, P, P, P, P, P, C.POINTER(I64)],
+        C.c_int,
+    ),
+    "tfm_pattern_device_fill": (
+        [P, P, I64, I32, I64, P, P, P, P, P, P, P, C.POINTER(I32)],
+        C.c_int,
+    ),
     "tfm_pattern_expand": ([P, C.POINTER(Mesh), P, P, P, P, P], C.c_int),
     "tfm_element_matrices": ([P, C.POINTER(Mesh), C.POINTER(Material), P, P], C.c_int),
     "tfm_assemble": (
-        [P, C.POINTER(Mesh), C.POINTER(Material), P, P, P, P, P],
+        [P, C.POINTER(Mesh), C.POINTER(Material), P, P, P, P, P, I32],
         C.c_int,
     ),
-    "tfm_assemble_elements": ([P, C.POINTER(Mesh), P, P, P, P, P], C.c_int),
+    "tfm_assemble_elements": ([P, C.POINTER(Mesh), P, P, P, P, P, I32], C.c_int),
     "tfm_internal_force": (
         [P, C.POINTER(Mesh), C.POINTER(Material), P, P, P, P, P, P, P, P],
         C.c_int,
@@ -137,6 +148,16 @@ _SIGNATURES: dict[str, tuple[list[Any], Any]] = {
     "tfm_pcg_timing": ([P, I32, C.POINTER(F64), C.POINTER(I64)], C.c_int),
     "tfm_csr_pcg_solve": (
         [P, P, I64, P, P, P, P, P, F64, C.POINTER(I32), C.POINTER(F64)],
+        C.c_int,
+    ),
+    "tfm_krylov_create": ([C.POINTER(P), I64, I32], C.c_int),
+    "tfm_krylov_destroy": ([P], C.c_int),
+    "tfm_csr_minres_solve": (
+        [P, P, I64, P, P, P, P, P, F64, I32, C.POINTER(I32), C.POINTER(F64)],
+        C.c_int,
+    ),
+    "tfm_csr_gmres_solve": (
+        [P, P, I64, P, P, P, P, P, F64, I32, C.POINTER(I32), C.POINTER(F64)],
         C.c_int,
     ),
     "tfm_facet_count": ([C.POINTER(Mesh), C.POINTER(I32), C.POINTER(I32)], C.c_int),

@@ -1,5 +1,7 @@
-"""Node-block sparsity pattern of a mesh, built once on the host by
-libtfm_hip.so (tfm_pattern_create) and kept on the device.
+"""Node-block sparsity pattern of a mesh, built once on the device by
+libtfm_hip.so (tfm_pattern_device_count/fill) and kept there. The host
+builder (tfm_pattern_create) produces the same arrays on the CPU; the tests
+use it where no GPU is present.
 
 This replaces the pattern block of FEM.__init__ (base.py:72-122): instead of
 sorting the packed (row << 32 | col) keys of every element DOF pair, the host
@@ -53,8 +55,48 @@ def build_host_pattern(elements: Tensor, n_nod: int, ndof: int) -> dict[str, np.
         lib.tfm_pattern_destroy(h)
 
 
+def build_device_pattern(elements: Tensor, n_nod: int, ndof: int, device) -> dict:
+    """Runs the device pattern builder on int32 device connectivity; returns
+    device tensors (uint16 epos in int16 storage) and the sizes."""
+    lib = L.lib()
+    el = elements
+    n_elem, npe = el.shape
+    i32 = dict(dtype=torch.int32, device=device)
+    out = {
+        "ne_ptr": torch.empty(n_nod + 1, **i32),
+        "ne_elem": torch.empty(max(n_elem * npe, 1), **i32),
+        "ne_loc": torch.empty(max(n_elem * npe, 1), **i32),
+        "adj_ptr": torch.empty(n_nod + 1, **i32),
+        "val_ptr": torch.empty(n_nod + 1, dtype=torch.int64, device=device),
+    }
+    work = torch.empty(3 * n_nod + 16 + 4 * (n_nod // 1024 + 2), **i32)
+    info = (L.I64 * 4)()
+    stream = L.stream_of(device)
+    L.check(lib.tfm_pattern_device_count(stream, L.ptr(el), n_elem, npe, n_nod, ndof,
+                                         L.ptr(out["ne_ptr"]), L.ptr(out["ne_elem"]),
+                                         L.ptr(out["ne_loc"]), L.ptr(out["adj_ptr"]),
+                                         L.ptr(out["val_ptr"]), L.ptr(work), info))
+    n_adj = int(info[0])
+    out["adj"] = torch.empty(max(n_adj, 1), **i32)
+    out["epos"] = torch.empty(max(n_elem * npe * npe, 1), dtype=torch.int16, device=device)
+    adj16 = torch.empty(max(n_adj, 1), dtype=torch.int16, device=device)
+    fits = L.I32()
+    L.check(lib.tfm_pattern_device_fill(stream, L.ptr(el), n_elem, npe, n_nod,
+                                        L.ptr(out["ne_ptr"]), L.ptr(out["ne_elem"]),
+                                        L.ptr(out["adj_ptr"]), L.ptr(out["adj"]),
+                                        L.ptr(out["epos"]), L.ptr(adj16), L.ptr(work),
+                                        C.byref(fits)))
+    out["adj16"] = adj16 if (fits.value and n_adj > 0) else None
+    out["nnz"] = int(info[1])
+    out["max_deg"] = int(info[2])
+    out["max_inc"] = int(info[3])
+    out["npe"] = int(npe)
+    return out
+
+
 class DevicePattern:
-    """Mesh geometry and pattern on the device, with the tfm_mesh_t view."""
+    """Mesh geometry and pattern on the device, with the tfm_mesh_t view. The
+    pattern is built on the device (tfm_pattern_device_count/fill)."""
 
     def __init__(self, nodes: Tensor, elements: Tensor, ndof: int, etype: str,
                  device: torch.device):
@@ -63,27 +105,22 @@ class DevicePattern:
         self.n_elem = elements.shape[0]
         self.ndof = ndof
         self.etype = etype
-        host = build_host_pattern(elements, self.n_nod, ndof)
-        self.nnz = host["nnz"]
-        self.max_deg = host["max_deg"]
-        self.max_inc = host["max_inc"]
-        self.npe = host["npe"] if self.n_elem else elements.shape[1]
         self.nodes = nodes.detach().to(device=device, dtype=torch.float64).contiguous()
         self.elements = elements.detach().to(device=device, dtype=torch.int32).contiguous()
-        up = lambda a: torch.from_numpy(a).to(device)  # noqa: E731
-        self.adj_ptr = up(host["adj_ptr"])
-        self.adj = up(host["adj"])
-        self.val_ptr = up(host["val_ptr"])
-        self.ne_ptr = up(host["ne_ptr"])
-        self.ne_elem = up(host["ne_elem"])
-        self.ne_loc = up(host["ne_loc"])
-        # uint16 travels as int16 storage (same bits)
-        self.epos = torch.from_numpy(host["epos"].view(np.int16)).to(device)
+        dev = build_device_pattern(self.elements, self.n_nod, ndof, device)
+        self.nnz = dev["nnz"]
+        self.max_deg = dev["max_deg"]
+        self.max_inc = dev["max_inc"]
+        self.npe = dev["npe"] if self.n_elem else elements.shape[1]
+        self.adj_ptr = dev["adj_ptr"]
+        self.adj = dev["adj"]
+        self.val_ptr = dev["val_ptr"]
+        self.ne_ptr = dev["ne_ptr"]
+        self.ne_elem = dev["ne_elem"]
+        self.ne_loc = dev["ne_loc"]
+        self.epos = dev["epos"]
         # 16-bit neighbour offsets (adj[k] - row node) when they all fit
-        rows = np.repeat(np.arange(self.n_nod, dtype=np.int64), np.diff(host["adj_ptr"]))
-        delta = host["adj"].astype(np.int64) - rows
-        fits = delta.size > 0 and int(np.abs(delta).max()) <= np.iinfo(np.int16).max
-        self.adj16 = up(delta.astype(np.int16)) if fits else None
+        self.adj16 = dev["adj16"]
         self.mesh = L.Mesh(
             nodes=self.nodes.data_ptr(),
             elements=self.elements.data_ptr(),

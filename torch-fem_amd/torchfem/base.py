@@ -52,7 +52,7 @@ class _System:
     diagonal blocks, the Dirichlet mask and the material it was built from."""
 
     def __init__(self, fem: "FEM", kind: int, p0: Tensor, p1: Tensor | None,
-                 scale: Tensor | None, precision: str):
+                 scale: Tensor | None, precision: str, nrhs: int = 1):
         pat = fem._pattern
         dev = pat.device
         self.fem = fem
@@ -72,14 +72,17 @@ class _System:
         nd = pat.ndof
         self.diag = torch.empty(pat.n_nod * nd * nd, dtype=torch.float64, device=dev)
         bad = torch.zeros(1, dtype=torch.int32, device=dev)
+        # FP32 values in difference form for one or two right-hand sides (the
+        # form the solver's FP32 products take for that many), plain for more
+        form = L.FP32_DIFF if nrhs <= 2 else L.FP32_PLAIN
         L.check(L.lib().tfm_assemble(pat.stream(), C.byref(pat.mesh), C.byref(self.mat),
                                      L.ptr(self.con), L.ptr(self.val64), L.ptr(self.val32),
-                                     L.ptr(self.diag), L.ptr(bad)))
+                                     L.ptr(self.diag), L.ptr(bad), form))
         fem._check_jacobian(bad)
         self.sys = L.System(mesh=C.pointer(pat.mesh), mat=C.pointer(self.mat),
                             con=self.con.data_ptr(),
                             val64=L.ptr(self.val64), val32=L.ptr(self.val32),
-                            diag_blocks=self.diag.data_ptr())
+                            diag_blocks=self.diag.data_ptr(), val32_form=form)
 
     def force(self, u: Tensor) -> Tensor:
         """K u with K unconstrained (matrix-free, FP64)."""
@@ -126,7 +129,7 @@ class _StaticSolve(torch.autograd.Function):
         F2 = F_ext.reshape(-1, n)
         nr = F2.shape[0]
         U2 = U_D.reshape(-1, n).expand(nr, n)
-        system = _System(fem, cfg["kind"], p0, p1, scale, cfg["precision"])
+        system = _System(fem, cfg["kind"], p0, p1, scale, cfg["precision"], nrhs=nr)
         con = system.con.bool()
         u0 = torch.where(con, U2, torch.zeros_like(U2))
         d = fem._newton(system, cfg, F2, u0, con)
@@ -528,7 +531,8 @@ class FEM(ABC):
         val = torch.empty(pat.nnz, dtype=torch.float64, device=self.device)
         diag = torch.empty(pat.n_nod * pat.ndof**2, dtype=torch.float64, device=self.device)
         L.check(L.lib().tfm_assemble_elements(pat.stream(), C.byref(pat.mesh), L.ptr(k),
-                                              L.ptr(mask), L.ptr(val), None, L.ptr(diag)))
+                                              L.ptr(mask), L.ptr(val), None, L.ptr(diag),
+                                              L.FP32_PLAIN))
         self.is_constrained = mask.bool()
         with torch.sparse.check_sparse_tensor_invariants(False):
             return torch.sparse_coo_tensor(self.glob_idx, val, size=(self.n_dofs, self.n_dofs),

@@ -7,11 +7,22 @@ pattern, `differentiable_sparse_solve` (246-268), `sparse_solve` (271-347),
 `resolve_method`/`describe_method` (74-100) and the generic Newton driver
 `NewtonRaphsonAdjoint`/`newton_solve` (517-795).
 
-The backend is libtfm_hip.so: "cg" (and "minres", which this backend maps to
-the same preconditioned CG, valid for the symmetric positive definite systems
-of finite-element models) runs the hand-written Jacobi-PCG; "spsolve" on small
-systems is a dense LU on the device. There is no CPU path: CPU tensors are
-moved to the GPU and the result is returned on the device of b.
+The backend is libtfm_hip.so, with Jacobi preconditioning throughout, as the
+reference's GPU path (sparse.py:406-421):
+
+* "cg": preconditioned CG for symmetric positive definite matrices; it stops
+  with an error as soon as an iteration meets p.Ap <= 0;
+* "minres": preconditioned MINRES for symmetric (possibly indefinite)
+  matrices; a non-symmetric matrix is refused;
+* "spsolve" (the default below 10,000 unknowns, sparse.py:74-84): the
+  reference factorises; here restarted GMRES runs to a relative residual of
+  1e-12, or to the attainable accuracy (at most 1e-10) when FP64 rounding
+  stalls it first, as it would a direct solve;
+* the default above 10,000 unknowns is "minres", as in the reference; a
+  non-symmetric matrix goes to GMRES instead.
+
+There is no CPU path: CPU tensors are moved to the GPU and the result is
+returned on the device of b.
 """
 
 from __future__ import annotations
@@ -30,7 +41,9 @@ from . import _lib as L
 
 available_backends = ["hip"]
 _DEFAULT_PRECISION = "mixed"
-_DENSE_LIMIT = 20000
+_DIRECT_RTOL = 1e-12   # relative residual "spsolve" aims at
+_DIRECT_ACCEPT = 1e-10  # ... and accepts when rounding stalls GMRES above it
+_GMRES_RESTART = 300   # basis vectors per GMRES cycle (fewer for small systems)
 
 
 def default_precision() -> str:
@@ -47,15 +60,15 @@ def set_default_precision(precision: str) -> None:
 
 def resolve_method(n_dofs: int, device: str, method: str | None) -> str:
     """Backend `sparse_solve` uses for a system of this size (sparse.py:74-84):
-    a direct solve below 10,000 DOFs, the iterative PCG above."""
+    "spsolve" below 10,000 DOFs, "minres" above (no AmgX in this build)."""
     if method is not None:
         return method
-    return "spsolve" if n_dofs < 10000 else "cg"
+    return "spsolve" if n_dofs < 10000 else "minres"
 
 
 def describe_method(n_dofs: int, device: str, method: str | None) -> str:
     resolved = resolve_method(n_dofs, device, method)
-    kind = "direct" if resolved == "spsolve" else "iterative | jacobi"
+    kind = "gmres to 1e-12" if resolved == "spsolve" else "iterative | jacobi"
     return f"{resolved} | {kind} | hip | cuda"
 
 
@@ -158,9 +171,56 @@ class PCGSolver:
                                        C.byref(its), C.byref(rel))
         info = dict(iterations=int(its.value), relres=float(rel.value))
         if rc == L.TFM_ERR_NOCONV:
-            raise RuntimeError(f"CG failed with exit code {info['iterations']}")
+            msg = L.lib().tfm_last_error().decode(errors="replace")
+            raise RuntimeError(f"CG failed after {info['iterations']} iterations: {msg}")
         L.check(rc)
         return info
+
+
+class KrylovSolver:
+    """Owner of a libtfm_hip MINRES / GMRES workspace (tfm_krylov_*), released
+    by `close()` like the reference's AmgXSolver (amgx.py:385-395)."""
+
+    def __init__(self, n: int, restart: int, device):
+        self.device = device
+        self.n = int(n)
+        h = L.P()
+        L.check(L.lib().tfm_krylov_create(C.byref(h), self.n, int(restart)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L.lib().tfm_krylov_destroy(self._h)
+            self._h = None
+
+    def solve(self, kind: str, rowptr, col, val, b, x, rtol: float, max_iter: int,
+              accept: float | None = None) -> dict:
+        """Runs MINRES or GMRES; a stop short of rtol is accepted when the true
+        relative residual is at most `accept` (the attainable accuracy)."""
+        its = L.I32()
+        rel = L.F64()
+        fn = L.lib().tfm_csr_minres_solve if kind == "minres" else L.lib().tfm_csr_gmres_solve
+        rc = fn(self._h, L.stream_of(self.device), self.n, L.ptr(rowptr), L.ptr(col), L.ptr(val),
+                L.ptr(b), L.ptr(x), float(rtol), int(max_iter), C.byref(its), C.byref(rel))
+        info = dict(iterations=int(its.value), relres=float(rel.value))
+        if rc == L.TFM_ERR_NOCONV:
+            if accept is not None and info["relres"] <= accept:
+                return info
+            msg = L.lib().tfm_last_error().decode(errors="replace")
+            raise RuntimeError(f"{kind} failed to converge (relative residual "
+                               f"{info['relres']:.3e}): {msg}")
+        L.check(rc)
+        return info
+
+
+def is_symmetric(A: Tensor, rtol: float = 1e-12) -> bool:
+    """Whether a sparse matrix equals its transpose up to rtol * max|A|."""
+    if A.layout == torch.sparse_csr:
+        A = A.to_sparse_coo()
+    A = A.coalesce()
+    vmax = float(A._values().abs().max()) if A._nnz() else 0.0
+    D = (A - A.t()).coalesce()
+    return D._nnz() == 0 or float(D._values().abs().max()) <= rtol * vmax
 
 
 def _csr_arrays(A: Tensor, device) -> tuple[Tensor, Tensor, Tensor]:
@@ -204,24 +264,42 @@ def sparse_solve(
     dev = L.require_gpu(device if device is not None else (b.device if b.is_cuda else None))
     out_device = b.device
     n = A.shape[0]
+    auto = method is None
     method = resolve_method(n, "cuda", method)
     if method in ("pardiso", "amgx"):
         raise RuntimeError(f"{method} is not available in this build (no CPU or NVIDIA backend).")
     bb = b.detach().to(device=dev, dtype=torch.float64).contiguous()
-    if method == "spsolve":
-        if n > _DENSE_LIMIT:
-            raise RuntimeError(f"spsolve on the GPU is a dense LU, limited to {_DENSE_LIMIT} DOFs")
-        Ad = A.detach().to(dev).to_dense().to(torch.float64)
-        x = torch.linalg.solve(Ad, bb)
-        return x.to(device=out_device, dtype=b.dtype), None
-    rowptr, col, val = _csr_arrays(A.detach(), dev)
+    Ad = A.detach().to(dev)
+    rowptr, col, val = _csr_arrays(Ad, dev)
     x = torch.zeros(n, dtype=torch.float64, device=dev) if x0 is None else (
         x0.detach().to(device=dev, dtype=torch.float64).clone().contiguous())
-    solver = PCGSolver(n, 1, dev)
+    if not bool(bb.any()):  # b = 0: x = 0 whatever the initial guess (as scipy's solvers)
+        x = torch.zeros(n, dtype=torch.float64, device=dev)
+        return x.to(device=out_device, dtype=b.dtype), (None if method == "spsolve" else (
+            M if M is not None else JacobiPreconditioner()))
+    if method == "minres" and not is_symmetric(Ad):
+        if not auto:
+            raise ValueError("minres needs a symmetric matrix; use method='spsolve' (GMRES)")
+        method = "gmres"  # the default for a large non-symmetric system
+    if method == "cg":
+        solver = PCGSolver(n, 1, dev)
+        try:
+            solver.solve_csr(rowptr, col, val, bb, x, stol)
+        finally:
+            solver.close()
+        return x.to(device=out_device, dtype=b.dtype), (M if M is not None else JacobiPreconditioner())
+    rtol = _DIRECT_RTOL if method == "spsolve" else stol
+    restart = min(n, _GMRES_RESTART)
+    ks = KrylovSolver(n, restart, dev)
     try:
-        solver.solve_csr(rowptr, col, val, bb, x, stol)
+        # "spsolve" returns the attainable accuracy when FP64 rounding stops
+        # GMRES short of 1e-12 (a direct solve has the same floor)
+        ks.solve("minres" if method == "minres" else "gmres", rowptr, col, val, bb, x, rtol,
+                 max(1000, 20 * n), accept=_DIRECT_ACCEPT if method == "spsolve" else None)
     finally:
-        solver.close()
+        ks.close()
+    if method == "spsolve":
+        return x.to(device=out_device, dtype=b.dtype), None
     return x.to(device=out_device, dtype=b.dtype), (M if M is not None else JacobiPreconditioner())
 
 

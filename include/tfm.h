@@ -199,6 +199,14 @@ int tfm_pattern_expand(tfm_stream_t s, const tfm_mesh_t *m, int64_t *rows,
 int tfm_element_matrices(tfm_stream_t s, const tfm_mesh_t *m,
                          const tfm_material_t *mat, double *k, int32_t *bad_jac);
 
+/* Element matrices from a tangent given per Gauss point,
+ * tangent [n_gp, n_elem, dim, dim, dim, dim] (a finite-strain material's
+ * ddsdde = dP/dF, materials/hyperelasticity.py:83-132): the BCB sum of
+ * Mechanics.integrate_material (base.py:1086-1090) per point, times
+ * scale [n_elem] (NULL = 1). Mechanics meshes only. */
+int tfm_element_matrices_gp(tfm_stream_t s, const tfm_mesh_t *m, const double *tangent,
+                            const double *scale, double *k, int32_t *bad_jac);
+
 /* Fused integration + deterministic assembly + Dirichlet epilogue: writes the
  * CSR values of K in FP64 and/or FP32 (either may be NULL) and the nodal
  * diagonal blocks. Replaces integrate_material's k plus

@@ -30,6 +30,7 @@ __all__ = [
     "cube_hexa", "rect_quad", "cube_tetra6", "linear_to_quadratic",
     "build_pattern", "iso3d_C", "plane_stress_C", "iso_KAPPA",
     "OracleModel", "simp_compliance_sensitivity", "hexa_nnz_closed_form",
+    "neo_hookean_psi", "hyper_step", "oracle_integrate_hyper", "oracle_solve_hyper",
 ]
 
 
@@ -758,3 +759,113 @@ def simp_compliance_sensitivity(model: OracleModel, u: Tensor, rho: Tensor, E0: 
     k = model.k0()  # model.mat holds the unit-modulus tangent
     ue = u.reshape(-1)[model.idx]
     return -p * rho ** (p - 1) * E0 * torch.einsum("ei,eij,ej->e", ue, k, ue)
+
+
+# --------------------------------------------------------------------------
+# Hyperelastic materials and the nonlinear solve
+# (src/torchfem/materials/hyperelasticity.py:12-140; base.py:597-923, 982-1092)
+# --------------------------------------------------------------------------
+def neo_hookean_psi(F: Tensor, params: Tensor) -> Tensor:
+    """Compressible Neo-Hookean energy psi = mu/2 (tr C - 3) - mu ln J +
+    lam/2 (ln J)^2 with params = (mu, lam) and ln J = 1/2 logdet(C), as the
+    reference's own test writes it (tests/test_gradients.py:89-94; its second
+    derivatives are finite at F = I, where those of det's SVD-based backward
+    are not)."""
+    mu, lam = params[0], params[1]
+    C = F.T @ F
+    lnJ = 0.5 * torch.logdet(C)
+    return 0.5 * mu * (torch.trace(C) - 3.0) - mu * lnJ + 0.5 * lam * lnJ**2
+
+
+def hyper_step(psi, F: Tensor, params: Tensor):
+    """Hyperelastic3D.step (hyperelasticity.py:83-132): the first
+    Piola-Kirchhoff stress P = dpsi/dF and the tangent d2psi/dF2 by automatic
+    differentiation, for a batch of deformation gradients [..., 3, 3] and
+    per-item parameters [..., p]."""
+    from torch.func import jacrev, vmap
+
+    lead = F.shape[:-2]
+    Ff = F.reshape(-1, 3, 3)
+    pf = params.expand(*lead, params.shape[-1]).reshape(-1, params.shape[-1])
+    with torch.enable_grad():
+        P = vmap(jacrev(psi))(Ff, pf)
+        A = vmap(jacrev(jacrev(psi)))(Ff, pf)
+    return P.reshape(*lead, 3, 3), A.reshape(*lead, 3, 3, 3, 3)
+
+
+def oracle_integrate_hyper(model: "OracleModel", psi, params: Tensor, grad_prev: Tensor,
+                           du: Tensor, need_k: bool = True):
+    """Mechanics.integrate_material for a hyperelastic material
+    (base.py:1036-1092): per Gauss point H = du B^T, F = grad_prev + H,
+    (P, A) = step, f += w detJ B^T P, k += w detJ B^T A B. params [E, p]."""
+    _, B, detJ = model.eval_shape_functions(model.etype.ipoints)
+    E, npe = model.n_elem, model.etype.nodes
+    due = du.view(-1, 3)[model.elements].reshape(E, -1, 3).transpose(-1, -2)
+    f = torch.zeros(E, 3 * npe, dtype=DT)
+    k = torch.zeros(E, 3 * npe, 3 * npe, dtype=DT) if need_k else None
+    Fs, Ps = [], []
+    for i, w in enumerate(model.etype.iweights):
+        H = due @ B[i].transpose(-1, -2)
+        F = grad_prev[i] + H
+        P, A = hyper_step(psi, F, params)
+        Fs.append(F)
+        Ps.append(P)
+        fc = torch.einsum("..., ...iI,...Ai->...IA", detJ[i], B[i], P)
+        f = f + float(w) * fc.reshape(-1, 3 * npe)
+        if need_k:
+            BCB = torch.einsum("...Jp,...iJkL,...Lq->...piqk", B[i], A, B[i])
+            k = k + torch.einsum("j,jkl->jkl", detJ[i], BCB.reshape(-1, 3 * npe, 3 * npe)) * float(w)
+    return k, f, torch.stack(Fs), torch.stack(Ps)
+
+
+def oracle_solve_hyper(model: "OracleModel", psi, params: Tensor, constraints: Tensor,
+                       forces: Tensor, displacements: Tensor, increments, nlgeom: bool = True,
+                       rtol: float = 1e-10, atol: float = 1e-10, max_iter: int = 30):
+    """FEM.solve's incremental Newton loop (base.py:700-880) for a
+    hyperelastic material, without cutbacks: per increment n the loads are
+    lam_n F_ext and (lam_n - lam_{n-1}) U_D, Newton starts from the previous
+    increment (du = du_{n-1}), K is reassembled every iteration with nlgeom
+    and kept from the first evaluation without it (base.py:1031-1033), and
+    du -= K^{-1} R until |R| < rtol |R_0| or |R| < atol (sparse.py:572-623).
+    Returns the final u, f (internal forces incl. reactions), the Gauss-point
+    mean of the stress (Cauchy F P / J with nlgeom, else P) and of F."""
+    con = torch.nonzero(constraints.ravel(), as_tuple=False).ravel()
+    n = model.n_dofs
+    grad = torch.eye(3, dtype=DT).expand(len(model.etype.iweights), model.n_elem, 3, 3).clone()
+    u = torch.zeros(n, dtype=DT)
+    du = torch.zeros(n, dtype=DT)
+    K = None
+    lam_prev = float(increments[0])
+    f_int = torch.zeros(n, dtype=DT)
+    P_last = None
+    for lam_n in [float(v) for v in increments[1:]]:
+        F_ext = lam_n * forces.ravel()
+        DU = (lam_n - lam_prev) * displacements.ravel()
+        res0 = None
+        for it in range(max_iter):
+            du_bc = du.clone()
+            du_bc[con] = DU[con]
+            need_k = K is None or nlgeom
+            k, fe, _, _ = oracle_integrate_hyper(model, psi, params, grad, du_bc, need_k)
+            if need_k:
+                K = model.csr(model.assemble_matrix(k, con))
+            R = model.assemble_rhs(fe) - F_ext
+            R[con] = 0.0
+            rn = float(torch.linalg.norm(R))
+            res0 = rn if res0 is None else res0
+            if rn < rtol * res0 or rn < atol:
+                break
+            du = du - torch.as_tensor(spla.spsolve(K.tocsc(), R.numpy()), dtype=DT)
+        else:
+            raise RuntimeError("Newton-Raphson iteration did not converge.")
+        du[con] = DU[con]
+        _, fe, grad, P_last = oracle_integrate_hyper(model, psi, params, grad, du, need_k=False)
+        f_int = model.assemble_rhs(fe)
+        u = u + du
+        lam_prev = lam_n
+    if nlgeom:
+        J = torch.linalg.det(grad)[..., None, None]
+        flux = grad @ P_last / J
+    else:
+        flux = P_last
+    return u.reshape(-1, 3), f_int.reshape(-1, 3), flux.mean(0), grad.mean(0)

@@ -110,3 +110,25 @@ def test_hexa_cube_64_pattern_size_closed_form():
     hp = build_host_pattern(el, len(nodes), 3)
     assert hp["nnz"] == O.hexa_nnz_closed_form(65) == 64_701_513
     assert hp["max_deg"] == 27 and hp["max_inc"] == 8
+
+
+def test_hyperelastic_material_matches_oracle():
+    """Hyperelastic3D's stress and tangent (torch.func on the host side of the
+    finite-strain path) against the oracle's hyper_step, per-element
+    parameters kept differentiable through vectorize."""
+    from torchfem.materials import Hyperelastic3D
+
+    g = torch.Generator().manual_seed(7)
+    F = torch.eye(3, dtype=torch.float64) + 0.2 * torch.randn(2, 4, 3, 3, generator=g,
+                                                               dtype=torch.float64)
+    base = torch.tensor([350.0, 800.0], dtype=torch.float64, requires_grad=True)
+    mat = Hyperelastic3D(O.neo_hookean_psi, base).vectorize(4)
+    assert mat.params.shape == (4, 2) and mat.is_vectorized
+    P, A = mat.stress_tangent(F)
+    P_o, A_o = O.hyper_step(O.neo_hookean_psi, F, base.detach())
+    assert torch.allclose(P, P_o, rtol=1e-13, atol=1e-10)
+    assert torch.allclose(A, A_o, rtol=1e-13, atol=1e-10)
+    assert torch.allclose(mat.stress(F), P_o, rtol=1e-13, atol=1e-10)
+    # P is linear in (mu, lam): dP/dparams . params == P
+    (gp,) = torch.autograd.grad(mat.stress(F).sum(), base)
+    assert abs(float(gp @ base.detach()) - float(P_o.sum())) < 1e-9 * float(P_o.abs().sum())

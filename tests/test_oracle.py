@@ -411,3 +411,128 @@ def test_surface_load_totals():
     assert conn.shape == (4, 8)
     assert torch.allclose(load.sum(0), torch.tensor([0.3, -0.2, 1.0], dtype=DT), atol=1e-13)
     assert (load[~top] == 0).all()
+
+
+# ------------------------------------------------------------ hyperelasticity
+def _nh_P_closed(F, lam, mu):
+    Fit = torch.linalg.inv(F).transpose(-1, -2)
+    lnJ = torch.log(torch.linalg.det(F))[..., None, None]
+    return mu * (F - Fit) + lam * lnJ * Fit
+
+
+def test_neo_hookean_stress_and_tangent():
+    """hyper_step's automatic derivatives of the Neo-Hookean energy against
+    P = mu (F - F^-T) + lam ln J F^-T, the tangent against central
+    differences of that closed form, major symmetry, and the small-strain
+    limit: at F = I the tangent is the isotropic C with G = mu."""
+    g = torch.Generator().manual_seed(0)
+    F = torch.eye(3, dtype=DT) + 0.2 * torch.randn(5, 3, 3, generator=g, dtype=DT)
+    lam, mu = 800.0, 350.0
+    params = torch.tensor([mu, lam], dtype=DT)
+    P, A = O.hyper_step(O.neo_hookean_psi, F, params)
+    assert torch.allclose(P, _nh_P_closed(F, lam, mu), rtol=1e-12, atol=1e-10)
+    h = 1e-6
+    for k in range(3):
+        for L_ in range(3):
+            dF = torch.zeros(3, 3, dtype=DT)
+            dF[k, L_] = h
+            fd = (_nh_P_closed(F + dF, lam, mu) - _nh_P_closed(F - dF, lam, mu)) / (2 * h)
+            assert torch.allclose(A[..., k, L_], fd, rtol=1e-6, atol=1e-5)
+    assert torch.allclose(A, A.permute(0, 3, 4, 1, 2), atol=1e-9)
+    _, A0 = O.hyper_step(O.neo_hookean_psi, torch.eye(3, dtype=DT)[None], params)
+    E = mu * (3 * lam + 2 * mu) / (lam + mu)
+    nu = lam / (2 * (lam + mu))
+    C = O.iso3d_C(torch.tensor([E], dtype=DT), torch.tensor(nu, dtype=DT))
+    assert torch.allclose(A0, C, rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("nlgeom", [False, True])
+def test_hyperelastic_affine_deformation_is_exact(nlgeom):
+    """An affine large deformation x = F0 X prescribed on the whole boundary
+    of a distorted Hexa1 mesh is reproduced inside (the elements contain
+    linear fields and a uniform P is in equilibrium), in two increments; the
+    stress is P(F0) (nlgeom: Cauchy F0 P / det F0) in every element."""
+    nodes, el = O.cube_hexa(4, 4, 4)
+    inner = ((nodes > 0) & (nodes < 1)).all(dim=1)
+    nodes = nodes + 0.05 * torch.sin(4.0 * nodes[:, [1, 2, 0]]) * inner[:, None]
+    m = O.OracleModel(nodes, el, O.Hexa1, "solid", O.iso3d_C(torch.full((len(el),), 1.0), 0.3))
+    F0 = torch.tensor([[1.25, 0.10, 0.0], [0.05, 0.90, 0.08], [0.0, -0.06, 1.10]], dtype=DT)
+    u_exact = nodes @ (F0 - torch.eye(3, dtype=DT)).T
+    bnd = _boundary(nodes)
+    con = bnd[:, None].repeat(1, 3)
+    params = torch.tensor([350.0, 800.0], dtype=DT).expand(len(el), 2)
+    u, f, flux, grad = O.oracle_solve_hyper(m, O.neo_hookean_psi, params, con,
+                                            torch.zeros_like(nodes), u_exact,
+                                            torch.tensor([0.0, 0.5, 1.0]), nlgeom=nlgeom)
+    assert torch.allclose(u, u_exact, atol=1e-12)
+    assert torch.allclose(grad, F0.expand_as(grad), atol=1e-12)
+    P0 = _nh_P_closed(F0, 800.0, 350.0)
+    expect = F0 @ P0 / torch.linalg.det(F0) if nlgeom else P0
+    assert torch.allclose(flux, expect.expand_as(flux), rtol=1e-11, atol=1e-9)
+    assert f[~bnd].abs().max() < 1e-9
+
+
+def test_hyperelastic_small_load_matches_linear_elasticity():
+    """For a small load the Neo-Hookean solution approaches the linear
+    elastic one with the same Lame constants (difference second order)."""
+    nodes, el = O.cube_hexa(3, 3, 3)
+    lam, mu = 800.0, 350.0
+    E = mu * (3 * lam + 2 * mu) / (lam + mu)
+    nu = lam / (2 * (lam + mu))
+    m = O.OracleModel(nodes, el, O.Hexa1, "solid", O.iso3d_C(torch.full((len(el),), E), nu))
+    con = torch.zeros(len(nodes), 3, dtype=torch.bool)
+    con[nodes[:, 2] == 0.0] = True
+    F = torch.zeros(len(nodes), 3, dtype=DT)
+    F[nodes[:, 2] == 1.0, 0] = 1e-3
+    params = torch.tensor([mu, lam], dtype=DT).expand(len(el), 2)
+    u_lin, _, _, _ = m.solve(con, F, torch.zeros_like(F))
+    errs = []
+    for s in (1.0, 0.1):
+        u_nl, _, _, _ = O.oracle_solve_hyper(m, O.neo_hookean_psi, params, con, s * F,
+                                             torch.zeros_like(F), torch.tensor([0.0, 1.0]),
+                                             nlgeom=True)
+        errs.append(float((u_nl - s * u_lin).norm() / (s * u_lin).norm()))
+    assert errs[0] < 1e-2
+    assert errs[1] < 0.2 * errs[0]  # the nonlinear part shrinks with the load
+
+
+def uniaxial_stretch_closed_form(mu, lbd, stretch):
+    """tests/test_gradients.py:114-125: the lateral stretch of a uniaxially
+    stretched Neo-Hookean block from mu (J / lam - 1) + lbd ln J = 0 (Newton),
+    the nominal reaction R = mu (lam - 1/lam) + lbd ln J / lam and its
+    derivatives with respect to (mu, lbd) by implicit differentiation."""
+    lam = stretch
+    J = 1.0
+    for _ in range(100):
+        J -= (mu * (J / lam - 1.0) + lbd * math.log(J)) / (mu / lam + lbd / J)
+    den = mu / lam + lbd / J
+    dJ_dmu = -(J / lam - 1.0) / den
+    dJ_dlbd = -math.log(J) / den
+    R = mu * (lam - 1.0 / lam) + lbd * math.log(J) / lam
+    dR = (lam - 1.0 / lam) + lbd / (lam * J) * dJ_dmu, math.log(J) / lam + lbd / (lam * J) * dJ_dlbd
+    return R, dR
+
+
+def test_hyperelastic_uniaxial_stretch_reaction():
+    """The reference's known answer (tests/test_gradients.py:80-128): a unit
+    cube of Neo-Hookean material stretched to 3x its length (8 geometric
+    increments, nlgeom) carries the analytical nominal reaction force."""
+    E, nu, U = 1000.0, 0.3, 2.0
+    lbd = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    mu = E / (2.0 * (1.0 + nu))
+    nodes, el = O.cube_hexa(3, 3, 3)
+    m = O.OracleModel(nodes, el, O.Hexa1, "solid", O.iso3d_C(torch.full((len(el),), 1.0), 0.3))
+    right = nodes[:, 0] == 1.0
+    con = torch.zeros(len(nodes), 3, dtype=torch.bool)
+    con[nodes[:, 0] == 0.0, 0] = True
+    con[right, 0] = True
+    con[nodes[:, 1] == 0.5, 1] = True
+    con[nodes[:, 2] == 0.5, 2] = True
+    D = torch.zeros(len(nodes), 3, dtype=DT)
+    D[right, 0] = U
+    lam_pts = torch.logspace(0, math.log10(1.0 + U), 8, dtype=DT)
+    params = torch.tensor([mu, lbd], dtype=DT).expand(len(el), 2)
+    u, f, _, _ = O.oracle_solve_hyper(m, O.neo_hookean_psi, params, con, torch.zeros_like(D), D,
+                                      (lam_pts - 1.0) / U, nlgeom=True)
+    R, _ = uniaxial_stretch_closed_form(mu, lbd, 1.0 + U)
+    assert abs(float(f[right, 0].sum()) - R) < 1e-8 * R

@@ -173,6 +173,60 @@ __global__ void __launch_bounds__(64) k_element_matrices(tfm_mesh_t m, tfm_mater
   }
 }
 
+// Element matrices from a tangent given per Gauss point, C[g][e][i][J][k][L]
+// (a finite-strain material's dP/dF): k = sum_g w detJ B^T C_g B times the
+// optional volume scale. The element's eight (or fewer) tangents are staged
+// in LDS next to its J^{-1} and w*detJ.
+template <class E>
+__global__ void __launch_bounds__(64) k_element_matrices_gp(tfm_mesh_t m, const double *cgp,
+                                                            const double *scale, double *kout,
+                                                            int32_t *bad) {
+  constexpr int NPE = E::NPE, NGP = E::NGP, D = E::DIM, NC = D * D * D * D;
+  constexpr int W = NPE * D, ND2 = D * D;
+  __shared__ double X[NPE][D];
+  __shared__ double Gm[NGP][D * D + 1];
+  __shared__ double Cg[NGP][NC];
+  const int64_t e = blockIdx.x;
+  const int tid = threadIdx.x;
+  for (int t = tid; t < NPE * D; t += 64) {
+    const int j = t / D, d = t - j * D;
+    X[j][d] = m.nodes[(int64_t)m.elements[e * NPE + j] * D + d];
+  }
+  for (int t = tid; t < NGP * NC; t += 64) {
+    const int g = t / NC, c = t - g * NC;
+    Cg[g][c] = cgp[((int64_t)g * m.n_elem + e) * NC + c];
+  }
+  __syncthreads();
+  const double sc = scale ? scale[e] : 1.0;
+  for (int g = tid; g < NGP; g += 64) {
+    double Ji[D][D];
+    const double det = geometry<E>(X, g, Ji);
+    if (!(det > 0.0)) *bad = 1;
+    for (int i = 0; i < D; ++i)
+      for (int k = 0; k < D; ++k) Gm[g][i * D + k] = Ji[i][k];
+    Gm[g][D * D] = shape_table<E>().w[g] * det * sc;
+  }
+  __syncthreads();
+  for (int t = tid; t < NPE * NPE; t += 64) {
+    const int I = t / NPE, J = t - I * NPE;
+    double blk[ND2];
+    for (int q = 0; q < ND2; ++q) blk[q] = 0.0;
+    for (int g = 0; g < NGP; ++g) {
+      double Ji[D][D];
+      for (int i = 0; i < D; ++i)
+        for (int k = 0; k < D; ++k) Ji[i][k] = Gm[g][i * D + k];
+      double bI[D], bJ[D], c[ND2];
+      bcol<E>(Ji, g, I, bI);
+      bcol<E>(Ji, g, J, bJ);
+      coupling_block<D, D, TFM_MAT_GENERAL>(bI, bJ, Cg[g], Gm[g][D * D], c);
+      for (int q = 0; q < ND2; ++q) blk[q] += c[q];
+    }
+    for (int a = 0; a < D; ++a)
+      for (int b = 0; b < D; ++b)
+        kout[e * W * W + (int64_t)(I * D + a) * W + J * D + b] = blk[a * D + b];
+  }
+}
+
 // ================================================================ assembly
 // One 64-lane block per node n assembles the node's ndof CSR rows: the
 // incident elements are visited in ascending order in chunks of KCH; each
@@ -714,6 +768,18 @@ int launch_internal_force(hipStream_t s, const tfm_mesh_t &m, const tfm_material
                          s, m, work, f);
       TFM_CHECK_LAUNCH();
     }
+  });
+  return TFM_OK;
+}
+
+int launch_element_matrices_gp(hipStream_t s, const tfm_mesh_t &m, const double *cgp,
+                               const double *scale, double *k, int32_t *bad) {
+  if (m.n_elem == 0) return TFM_OK;
+  TFM_REQUIRE(m.ndof == m.dim, "a per-point tangent needs a mechanics mesh");
+  TFM_DISPATCH(m.etype, {
+    hipLaunchKernelGGL((k_element_matrices_gp<ET>), dim3((unsigned)m.n_elem), dim3(64), 0, s, m,
+                       cgp, scale, k, bad);
+    TFM_CHECK_LAUNCH();
   });
   return TFM_OK;
 }

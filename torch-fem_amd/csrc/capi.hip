@@ -224,6 +224,14 @@ extern "C" int tfm_element_matrices(tfm_stream_t s, const tfm_mesh_t *m,
   return launch_element_matrices((hipStream_t)s, *m, *mat, k, bad_jac);
 }
 
+extern "C" int tfm_element_matrices_gp(tfm_stream_t s, const tfm_mesh_t *m,
+                                       const double *tangent, const double *scale, double *k,
+                                       int32_t *bad_jac) {
+  TFM_TRY(check_mesh(m));
+  TFM_REQUIRE(tangent && k && bad_jac, "null argument");
+  return launch_element_matrices_gp((hipStream_t)s, *m, tangent, scale, k, bad_jac);
+}
+
 extern "C" int tfm_assemble(tfm_stream_t s, const tfm_mesh_t *m, const tfm_material_t *mat,
                             const uint8_t *con, double *val64, float *val32,
                             double *diag_blocks, int32_t *bad_jac, int32_t fp32_form) {

@@ -273,6 +273,8 @@ int launch_internal_force(hipStream_t s, const tfm_mesh_t &m, const tfm_material
                           const double *u, const double *flux_prev, double *f,
                           double *grad_mean, double *flux_mean, double *grad_gp,
                           double *flux_gp, double *work);
+int launch_element_matrices_gp(hipStream_t s, const tfm_mesh_t &m, const double *cgp,
+                               const double *scale, double *k, int32_t *bad);
 int launch_assemble_elements(hipStream_t s, const tfm_mesh_t &m, const double *k,
                              const uint8_t *con, double *v64, float *v32, double *dblk,
                              bool diff32);

@@ -36,6 +36,7 @@ from . import _lib as L
 from ._pattern import DevicePattern
 from .elements import Element
 from .materials import Material
+from ._finite import Kinematics, solve_finite
 from .sparse import CachedSolve, PCGSolver, default_precision
 
 _PREC = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED}
@@ -498,9 +499,24 @@ class FEM(ABC):
     def integrate_material(self, u_prev, grad_prev, flux_prev, state_prev, du, de0, iter, nlgeom,
                            compute_stiffness: bool = True):
         """Element stiffness, forces, gradients and fluxes for the increment du
-        from a previous state (base.py:247-278, 982-1092, 1177-1286)."""
+        from a previous state (base.py:247-278, 982-1092, 1177-1286). A
+        hyperelastic material integrates F = grad_prev + grad(du) with its
+        stress and tangent per Gauss point; the flux is then the Cauchy stress
+        F P / J with nlgeom and P without it."""
+        if getattr(self.material, "finite_strain", False):
+            kin = Kinematics(self)
+            du = torch.as_tensor(du).reshape(-1).to(self.device, torch.float64)
+            F = grad_prev.to(self.device, torch.float64) + kin.grad(du)
+            if compute_stiffness:
+                P, A = self.material.stress_tangent(F)
+                k = kin.matrices(A)
+            else:
+                P, k = self.material.stress(F), None
+            f = kin.element_forces(P)
+            flux = F @ P / torch.linalg.det(F)[..., None, None] if nlgeom else P
+            return k, f, F, flux, state_prev.clone()
         if nlgeom:
-            raise NotImplementedError("nlgeom is not part of this build")
+            raise NotImplementedError("nlgeom is supported for hyperelastic materials only")
         if de0 is not None and torch.as_tensor(de0).abs().max() > 0:
             raise NotImplementedError("external gradient increments are not part of this build")
         kind, p0, p1, scale = self._material_params()
@@ -713,8 +729,11 @@ class FEM(ABC):
         ("block_jacobi" or "jacobi") and `inner_rtol` (mixed precision: the factor by
         which the FP32 recursive residual drops between two reliable updates).
         """
-        if nlgeom:
-            raise NotImplementedError("nlgeom (geometric nonlinearity) is not part of this build")
+        finite = getattr(self.material, "finite_strain", False)
+        if nlgeom and not finite:
+            raise NotImplementedError(
+                "nlgeom is supported for hyperelastic materials (Hyperelastic3D) in this build"
+            )
         if alpha > 0.0:
             raise NotImplementedError("viscous stabilization is not part of this build")
         if self.n_state != 0:
@@ -748,6 +767,19 @@ class FEM(ABC):
         else:
             params = tuple(differentiable_parameters)
         track = any(p.requires_grad for p in params)
+
+        if finite:
+            cfg = dict(precision=precision, pc=preconditioner, stol=stol, rtol=rtol, atol=atol,
+                       max_iter=max_iter, inner_rtol=inner_rtol)
+            self._K_cache = None
+            u_all, f_all, q_all, g_all, s_all = solve_finite(
+                self, incs, cfg, nlgeom, params, track, cutback_factor, growth_factor,
+                max_cutbacks, aggregate_integration_points, verbose)
+            if not track:
+                u_all, f_all, q_all, g_all = (t.detach() for t in (u_all, f_all, q_all, g_all))
+            if return_intermediate:
+                return u_all, f_all, q_all, g_all, s_all
+            return u_all[-1], f_all[-1], q_all[-1], g_all[-1], s_all[-1]
 
         kind, p0, p1, scale = self._material_params()
         cfg = dict(kind=kind, precision=precision, pc=preconditioner, stol=stol, rtol=rtol,

@@ -27,6 +27,7 @@ __all__ = [
     "IsotropicElasticityPlaneStress",
     "IsotropicConductivity3D",
     "IsotropicConductivity2D",
+    "Hyperelastic3D",
 ]
 
 
@@ -185,3 +186,75 @@ class IsotropicConductivity2D(IsotropicConductivity3D):
     """Planar isotropic heat conduction (conductivity.py, 2D variant)."""
 
     dim = 2
+
+
+class Hyperelastic3D(Material):
+    """Hyperelastic material in 3D (hyperelasticity.py:12-127): a strain
+    energy density psi(F, params) supplies the first Piola-Kirchhoff stress
+    P = dpsi/dF and the tangent A = d2psi/dF2, both by torch.func automatic
+    differentiation, per Gauss point. It has no constant tangent, so models
+    with it are solved by the finite-strain Newton path of FEM.solve, whose
+    displacement gradients, internal forces and element matrices are HIP
+    kernels; psi itself runs as the user wrote it."""
+
+    dim = 3
+    finite_strain = True
+
+    def __init__(self, psi, params, rho: Tensor | float = 1.0):
+        self.psi = psi
+        self.params = torch.as_tensor(params)
+        self.n_state = 0
+        self.rho = _t(rho)
+        self.is_vectorized = self.params.dim() > 1
+
+    def vectorize(self, n_elem: int) -> "Hyperelastic3D":
+        """Parameters repeated per element, [n_elem, p] (hyperelasticity.py:73-80
+        with base.py:29-46); the autograd link to the given tensor is kept."""
+        if self.is_vectorized:
+            return self
+        new = copy.copy(self)
+        new.params = self.params.repeat(n_elem, 1)
+        new.rho = self.rho.repeat(n_elem)
+        new.is_vectorized = True
+        return new
+
+    def _batched(self, F: Tensor, params: Tensor | None):
+        p = self.params if params is None else params
+        p = p.to(device=F.device, dtype=torch.float64)
+        lead = F.shape[:-2]
+        # (p,) or [n_elem, p] against F [..., n_elem, 3, 3]
+        p = p.expand(*lead, p.shape[-1])
+        return F.reshape(-1, 3, 3), p.reshape(-1, p.shape[-1]), lead
+
+    def stress(self, F: Tensor, params: Tensor | None = None) -> Tensor:
+        """P(F) for deformation gradients [..., 3, 3]."""
+        from torch.func import jacrev, vmap
+
+        Ff, pf, lead = self._batched(F, params)
+        P = vmap(jacrev(self.psi))(Ff, pf)
+        return P.reshape(*lead, 3, 3)
+
+    def stress_tangent(self, F: Tensor, params: Tensor | None = None):
+        """(P, A) with A[..., i, J, k, L] = d2psi / dF_iJ dF_kL."""
+        from torch.func import jacrev, vmap
+
+        Ff, pf, lead = self._batched(F, params)
+        dpsi = jacrev(self.psi)
+
+        def both(f, q):
+            P = dpsi(f, q)
+            return P, P
+
+        A, P = vmap(jacrev(both, has_aux=True))(Ff, pf)
+        return P.reshape(*lead, 3, 3), A.reshape(*lead, 3, 3, 3, 3)
+
+    def step(self, H_inc, F, stress, state, de0, cl, iter):
+        """(P, state, A) at F + H_inc (hyperelasticity.py:83-127)."""
+        P, A = self.stress_tangent(F + H_inc)
+        return P, state, A
+
+    def kernel_params(self, n_elem, device):
+        raise TypeError(
+            "a hyperelastic material has no constant tangent: its models are solved "
+            "by the finite-strain Newton path of FEM.solve"
+        )

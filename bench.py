@@ -274,7 +274,8 @@ def other_workload(args) -> dict:
             rho = rho0.clone().requires_grad_(True)
             model.material = IsotropicElasticity3D(1000.0 * rho**3, 0.3)
             u, _, _, _, _ = model.solve(differentiable_parameters=rho, precision=precision,
-                                        stol=stol, preconditioner=args.preconditioner)
+                                        stol=stol, preconditioner=args.preconditioner,
+                                        inner_rtol=args.inner_rtol)
             (g,) = torch.autograd.grad(torch.inner(F.ravel(), u.ravel()), rho)
             return u.detach(), g
     elif args.workload == "tetra80":
@@ -294,7 +295,8 @@ def other_workload(args) -> dict:
             kap = k0.clone().requires_grad_(True)
             model.material = IsotropicConductivity3D(kap)
             T, _, _, _, _ = model.solve(differentiable_parameters=kap, precision=precision,
-                                        stol=stol, preconditioner=args.preconditioner)
+                                        stol=stol, preconditioner=args.preconditioner,
+                                        inner_rtol=args.inner_rtol)
             (g,) = torch.autograd.grad(T.mean(), kap)
             return T.detach(), g
     else:
@@ -319,7 +321,8 @@ def other_workload(args) -> dict:
             E = torch.full((model.n_elem,), 1000.0, dtype=dt, device="cuda", requires_grad=True)
             model.material = IsotropicElasticity3D(E, 0.3)
             u, _ = model.solve_load_cases(Fb, differentiable_parameters=E, precision=precision,
-                                          stol=stol, preconditioner=args.preconditioner)
+                                          stol=stol, preconditioner=args.preconditioner,
+                                          inner_rtol=args.inner_rtol)
             (g,) = torch.autograd.grad((Fb * u).sum(), E)
             return u.detach(), g
 
@@ -328,8 +331,10 @@ def other_workload(args) -> dict:
                                      "mixed")
     info = dict(model.last_solve_info)
     ms_fp64 = None
+    info64 = None
     if args.fp64_steps > 0:
         ms_fp64, _ = timed(lambda: step("fp64", args.stol), args.fp64_steps, 1, "fp64")
+        info64 = dict(model.last_solve_info)
     acc = {}
     if not args.no_reference:
         u_ref, g_ref = step("fp64", 1e-11)
@@ -348,12 +353,17 @@ def other_workload(args) -> dict:
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "n_dofs": model.n_dofs, "load_cases": nrhs,
-                   "nnz": model._pattern.nnz, "stol": args.stol,
-                   "preconditioner": args.preconditioner},
+                   "nnz": model._pattern.nnz, "n_adj": int(model._pattern.adj.numel()),
+                   "adj16": model._pattern.adj16 is not None, "stol": args.stol,
+                   "preconditioner": args.preconditioner, "inner_rtol": args.inner_rtol},
         "fp64_ms_per_step": ms_fp64 * 1e3 if ms_fp64 else None,
         "speedup_vs_fp64": ms_fp64 / ms_mixed if ms_fp64 else None,
-        "pcg_iterations": {"mixed_fwd": [sum(p["iterations"][r] for p in fwd) for r in range(nrhs)],
-                           "mixed_adj": info["adjoint"]["iterations"][:nrhs]},
+        "pcg_iterations": {
+            "mixed_fwd": [sum(p["iterations"][r] for p in fwd) for r in range(nrhs)],
+            "mixed_adj": info["adjoint"]["iterations"][:nrhs],
+            "fp64_fwd": ([sum(p["iterations"][r] for p in info64["forward"]["pcg"])
+                          for r in range(nrhs)] if info64 else None),
+            "fp64_adj": info64["adjoint"]["iterations"][:nrhs] if info64 else None},
         "accuracy": acc,
     }
 

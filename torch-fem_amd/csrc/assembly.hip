@@ -62,7 +62,7 @@ __device__ __forceinline__ int64_t row_start(const tfm_mesh_t &m, int64_t n, int
 
 __global__ void k_pattern_rows(tfm_mesh_t m, int64_t *rows, int64_t *cols, int64_t *rowptr,
                                int32_t *diag_map) {
-  const int64_t n = blockIdx.x;
+  const int64_t n = xcd_block();
   const int nd = m.ndof;
   const int a0 = m.adj_ptr[n];
   const int deg = m.adj_ptr[n + 1] - a0;
@@ -98,7 +98,7 @@ __global__ void k_pattern_rows(tfm_mesh_t m, int64_t *rows, int64_t *cols, int64
 
 // k_map[e, i*nd+a, j*nd+b] = slot of (row n_i*nd+a, col n_j*nd+b) (base.py:94-104)
 __global__ void k_pattern_kmap(tfm_mesh_t m, int32_t *k_map) {
-  const int64_t e = blockIdx.x;
+  const int64_t e = xcd_block();
   const int npe = m.npe, nd = m.ndof, W = npe * nd;
   for (int t = threadIdx.x; t < W * W; t += blockDim.x) {
     const int r = t / W, c = t - r * W;
@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(64) k_element_matrices(tfm_mesh_t m, tfm_mater
   __shared__ double X[NPE][D];
   __shared__ double Gm[NGP][D * D + 1];
   __shared__ double MP[NP];
-  const int64_t e = blockIdx.x;
+  const int64_t e = xcd_block();
   const int tid = threadIdx.x;
   for (int t = tid; t < NPE * D; t += 64) {
     const int j = t / D, d = t - j * D;
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(64) k_element_matrices_gp(tfm_mesh_t m, const 
   __shared__ double X[NPE][D];
   __shared__ double Gm[NGP][D * D + 1];
   __shared__ double Cg[NGP][NC];
-  const int64_t e = blockIdx.x;
+  const int64_t e = xcd_block();
   const int tid = threadIdx.x;
   for (int t = tid; t < NPE * D; t += 64) {
     const int j = t / D, d = t - j * D;
@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(64) k_assemble_rows(tfm_mesh_t m, tfm_material
   double *acc = dyn;                                              // [ND2 * maxdeg]
   signed char *slot = reinterpret_cast<signed char *>(acc + ND2 * m.max_deg);  // [KCH * maxdeg]
 
-  const int64_t n = blockIdx.x;
+  const int64_t n = xcd_block();
   const int tid = threadIdx.x;
   const int a0 = m.adj_ptr[n];
   const int deg = m.adj_ptr[n + 1] - a0;
@@ -423,12 +423,13 @@ __global__ void __launch_bounds__(256) k_internal_force(tfm_mesh_t m, tfm_materi
   constexpr int LG = team_lanes<E>(), EPB = 256 / LG, NQ = NDOF * D;
   __shared__ double MP[EPB][NP];
   const int tid = threadIdx.x, slotE = tid / LG, j = tid % LG;
-  const int64_t e = (int64_t)blockIdx.x * EPB + slotE;
+  const int64_t blk = xcd_block();
+  const int64_t e = blk * EPB + slotE;
   const bool live = e < m.n_elem;
   const bool act = live && j < NPE;
   for (int t = tid; t < EPB * NP; t += 256) {
     const int k = t / NP, i = t - k * NP;
-    const int64_t ee = (int64_t)blockIdx.x * EPB + k;
+    const int64_t ee = blk * EPB + k;
     MP[k][i] = ee < m.n_elem ? mat_param<MAT, D>(mat, ee, i) : 0.0;
   }
   double Xj[D], uj[NDOF];
@@ -484,7 +485,7 @@ __global__ void __launch_bounds__(256) k_internal_force(tfm_mesh_t m, tfm_materi
 // scatter of FEM.assemble_rhs (base.py:439-445) as an ordered gather.
 template <int NDOF>
 __global__ void k_gather_nodes(tfm_mesh_t m, const double *fe, double *F) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= m.n_nod * NDOF) return;
   const int64_t n = t / NDOF;
   const int a = (int)(t - n * NDOF);
@@ -516,7 +517,7 @@ __global__ void __launch_bounds__(64) k_assemble_elements(tfm_mesh_t m, const do
                                                           float *v32, double *dblk, int diff32) {
   constexpr int ND2 = NDOF * NDOF;
   extern __shared__ signed char slot_dyn[];  // [max_inc * max_deg]
-  const int64_t n = blockIdx.x;
+  const int64_t n = xcd_block();
   const int tid = threadIdx.x;
   const int npe = m.npe, W = npe * NDOF;
   const int a0 = m.adj_ptr[n];
@@ -591,12 +592,13 @@ __global__ void __launch_bounds__(256) k_sensitivity(tfm_mesh_t m, tfm_material_
   constexpr int PER = (NO - 1 + LG - 1) / LG;  // general layouts: outputs per lane
   __shared__ double MP[EPB][NP];
   const int tid = threadIdx.x, slotE = tid / LG, j = tid % LG;
-  const int64_t e = (int64_t)blockIdx.x * EPB + slotE;
+  const int64_t blk = xcd_block();
+  const int64_t e = blk * EPB + slotE;
   const bool live = e < m.n_elem;
   const bool act = live && j < NPE;
   for (int t = tid; t < EPB * NP; t += 256) {
     const int k = t / NP, i = t - k * NP;
-    const int64_t ee = (int64_t)blockIdx.x * EPB + k;
+    const int64_t ee = blk * EPB + k;
     MP[k][i] = ee < m.n_elem ? mat_param<MAT, D>(mat, ee, i) : 0.0;
   }
   double Xj[D];

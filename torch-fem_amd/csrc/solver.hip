@@ -534,6 +534,7 @@ struct PcgState {
   PcgCoef *coef_a;   // beta of A (scalar-kernel mode)
   PcgCoef *coef_b;   // alpha of B (scalar-kernel mode)
   int xcd;           // grid-stride kernels give each XCD one contiguous share (block_span)
+  int *segf;         // [NRP] iteration at which rhs r's current segment started
 };
 
 // Items [first, end) with step `stride` of a block of a grid-stride kernel,
@@ -567,7 +568,7 @@ struct SegInfo {
 // read at kernel entry, so that its load overlaps the partial loads) makes it
 // return 0 before recording anything.
 template <int NRP>
-__device__ __forceinline__ int dir_prologue(int it, int seg_first, const double *part_in, int n_in,
+__device__ __forceinline__ int dir_prologue(int it, const double *part_in, int n_in,
                                             const PcgState &st, double *sh, double *tot,
                                             double *beta, int *live, int ended = 0) {
   sum_partials<2 * NRP>(part_in, n_in, sh, tot);
@@ -580,7 +581,7 @@ __device__ __forceinline__ int dir_prologue(int it, int seg_first, const double 
     const bool was = st.done[tid] >= 0;
     const bool now = !was && (rr <= st.tol2[tid] || !(rr == rr));
     const bool lv = !was && !now;
-    const double den = it == seg_first ? st.rzold[tid]
+    const double den = it == st.segf[tid] ? st.rzold[tid]
                                        : st.scal[((size_t)(it - 1) * NRP + tid) * NSCAL + S_RZ];
     live[tid] = lv;
     beta[tid] = (lv && den != 0.0) ? g / den : 0.0;
@@ -606,7 +607,7 @@ __device__ __forceinline__ int dir_prologue(int it, int seg_first, const double 
 template <typename T, int ND, int TL, int NRP, bool CO, bool D16>
 __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__restrict__ val,
                                               const T *__restrict__ u, T *__restrict__ p,
-                                              T *__restrict__ s, int it, int seg_first,
+                                              T *__restrict__ s, int it,
                                               const double *part_in, int n_in,
                                               const PcgState &st) {
   static_assert(TL % NRP == 0, "team width must be a multiple of the rhs count");
@@ -636,7 +637,7 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
     __syncthreads();
   } else {
     const int ended = *st.alldone;
-    if (!dir_prologue<NRP>(it, seg_first, part_in, n_in, st, sh, tot, beta, live, ended)) return;
+    if (!dir_prologue<NRP>(it, part_in, n_in, st, sh, tot, beta, live, ended)) return;
   }
   // lane l updates the elements e = l, l + TL, ... of the node's ND*NRP
   // (offset e from the node's first entry); they all belong to rhs l % NRP
@@ -702,10 +703,10 @@ __device__ __forceinline__ void spmv_dir_body(const tfm_mesh_t &m, const T *__re
 template <typename T, int ND, int TL, int NRP, bool CO, bool D16>
 __global__ void __launch_bounds__(BS) k_pcg_spmv_dir(tfm_mesh_t m, const T *__restrict__ val,
                                                      const T *__restrict__ u, T *__restrict__ p,
-                                                     T *__restrict__ s, int it, int seg_first,
+                                                     T *__restrict__ s, int it,
                                                      const double *part_in, int n_in,
                                                      PcgState st) {
-  spmv_dir_body<T, ND, TL, NRP, CO, D16>(m, val, u, p, s, it, seg_first, part_in, n_in, st);
+  spmv_dir_body<T, ND, TL, NRP, CO, D16>(m, val, u, p, s, it, part_in, n_in, st);
 }
 
 // The same kernel held to 64 VGPRs (8 waves per SIMD), so that a grid of
@@ -713,18 +714,18 @@ __global__ void __launch_bounds__(BS) k_pcg_spmv_dir(tfm_mesh_t m, const T *__re
 template <typename T, int ND, int TL, int NRP, bool CO, bool D16>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_pcg_spmv_dir_w8(tfm_mesh_t m, const T *__restrict__ val, const T *__restrict__ u,
-                  T *__restrict__ p, T *__restrict__ s, int it, int seg_first,
+                  T *__restrict__ p, T *__restrict__ s, int it,
                   const double *part_in, int n_in, PcgState st) {
-  spmv_dir_body<T, ND, TL, NRP, CO, D16>(m, val, u, p, s, it, seg_first, part_in, n_in, st);
+  spmv_dir_body<T, ND, TL, NRP, CO, D16>(m, val, u, p, s, it, part_in, n_in, st);
 }
 
 // Held to 80 VGPRs (6 waves per SIMD), for kernels that spill at 64.
 template <typename T, int ND, int TL, int NRP, bool CO, bool D16>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6)))
 k_pcg_spmv_dir_w6(tfm_mesh_t m, const T *__restrict__ val, const T *__restrict__ u,
-                  T *__restrict__ p, T *__restrict__ s, int it, int seg_first,
+                  T *__restrict__ p, T *__restrict__ s, int it,
                   const double *part_in, int n_in, PcgState st) {
-  spmv_dir_body<T, ND, TL, NRP, CO, D16>(m, val, u, p, s, it, seg_first, part_in, n_in, st);
+  spmv_dir_body<T, ND, TL, NRP, CO, D16>(m, val, u, p, s, it, part_in, n_in, st);
 }
 
 // A over a general CSR matrix (one rhs).
@@ -732,13 +733,13 @@ template <int TL>
 __global__ void __launch_bounds__(BS) k_pcg_csr_dir(int64_t n, const int64_t *rowptr,
                                                     const int32_t *col, const double *val,
                                                     const double *u, double *p, double *s, int it,
-                                                    int seg_first, const double *part_in, int n_in,
+                                                    const double *part_in, int n_in,
                                                     PcgState st) {
   if (*st.alldone) return;
   __shared__ double sh[BS];
   __shared__ double tot[2], beta[1];
   __shared__ int live[1];
-  if (!dir_prologue<1>(it, seg_first, part_in, n_in, st, sh, tot, beta, live)) return;
+  if (!dir_prologue<1>(it, part_in, n_in, st, sh, tot, beta, live)) return;
   constexpr int TPB = BS / TL;
   const int lane = threadIdx.x % TL, team = threadIdx.x / TL;
   double dot = 0.0;
@@ -838,12 +839,12 @@ __global__ void __launch_bounds__(BSV) k_pcg_update(int64_t n_nod, int it, T *__
 // iteration `it`: the same scalars, stop test and records as the prologue of
 // A (dir_prologue, run here by a single block), published in coef_a.
 template <int NRP>
-__global__ void __launch_bounds__(BS) k_pcg_scal_dir(int it, int seg_first, const double *part_in,
+__global__ void __launch_bounds__(BS) k_pcg_scal_dir(int it, const double *part_in,
                                                      int n_in, PcgState st) {
   __shared__ double sh[BS];
   __shared__ double tot[2 * NRP], beta[NRP];
   __shared__ int live[NRP];
-  const int any = dir_prologue<NRP>(it, seg_first, part_in, n_in, st, sh, tot, beta, live);
+  const int any = dir_prologue<NRP>(it, part_in, n_in, st, sh, tot, beta, live);
   const int tid = threadIdx.x;
   if (tid < NRP) {
     st.coef_a->c[tid] = beta[tid];
@@ -917,16 +918,18 @@ __global__ void __launch_bounds__(BS) k_pcg_seg_dots(int64_t n_nod, const T *r, 
   write_block_partials<1, NRP>(v, sh, st.part_aux);
 }
 
-// Segment start, part 2. A rhs whose bit is set in `mask` runs the next
-// segment. If its bit is also set in `cont`, it keeps its search direction:
-// p' = p - (r.p / r.u) u is orthogonal to the new residual, and the first
-// beta of the segment divides by gamma of the iteration at which it stopped
-// (seg.d - 1); the caller then forms s = K p'. Otherwise it restarts from
-// p = s = 0 (beta = 0). A rhs outside `mask` has finished and is left as is.
+// Segment start, part 2. A rhs whose bit is set in `mask` starts its next
+// segment at iteration `it`. If its bit is also set in `cont`, it keeps its
+// search direction: p' = p - (r.p / r.u) u is orthogonal to the new residual,
+// and the first beta of the segment divides by gamma of the iteration at
+// which it stopped (seg.d - 1); the caller then forms s = K p'. Otherwise it
+// restarts from p = s = 0 (beta = 0). A rhs outside `mask` has either
+// finished or is still inside its own segment, and is left as is; `running`
+// holds every rhs that continues to iterate (mask included).
 template <typename T, int ND, int NRP>
 __global__ void __launch_bounds__(BS) k_pcg_seg_apply(int64_t n_nod, int it, int mask, int cont,
-                                                      SegInfo seg, const T *u, T *p, T *s,
-                                                      PcgState st, int nblk) {
+                                                      int running, SegInfo seg, const T *u, T *p,
+                                                      T *s, PcgState st, int nblk) {
   __shared__ double sh[BS];
   __shared__ double tot[2 * NRP], rp[NRP], cz[NRP];
   __shared__ int mode[NRP];
@@ -944,12 +947,18 @@ __global__ void __launch_bounds__(BS) k_pcg_seg_apply(int64_t n_nod, int it, int
     mode[tid] = keep ? 1 : (run ? 2 : 0);
     cz[tid] = keep ? c : 0.0;
     if (blockIdx.x == 0) {
-      st.rzold[tid] = keep ? den : 0.0;
-      st.done[tid] = run ? -1 : (d >= 0 ? d : it);
+      if (run) {
+        st.rzold[tid] = keep ? den : 0.0;
+        st.done[tid] = -1;
+        st.segf[tid] = it;
+      } else if (!((running >> tid) & 1)) {  // finished (or an unused slot): stopped
+        st.done[tid] = d >= 0 ? d : it;
+        st.segf[tid] = it;
+      }
     }
   }
   __syncthreads();
-  if (blockIdx.x == 0 && tid == 0) *st.alldone = mask == 0;
+  if (blockIdx.x == 0 && tid == 0) *st.alldone = running == 0;
   const int64_t total = n_nod * ND * NRP;
   for (int64_t t = (int64_t)blockIdx.x * BS + tid; t < total; t += (int64_t)gridDim.x * BS) {
     const int rr = (int)(t % NRP);
@@ -1237,13 +1246,13 @@ static int spmv_launch(hipStream_t s, const tfm_mesh_t &m, const T *val, const T
 
 template <typename T, int ND, int TL, int NRP, bool D16>
 static int spmv_dir_launch_d(hipStream_t s, const SpmvShape &sh, const tfm_mesh_t &m,
-                             const T *val, const T *u, T *p, T *sv, int it, int seg_first,
+                             const T *val, const T *u, T *p, T *sv, int it,
                              const double *part_in, int n_in, const PcgState &st, unsigned *g) {
   if (sh.scal) {
     *g = grid_for(m.n_nod, BS / TL, sh.cap);
     TFM_REQUIRE((int64_t)*g <= st.pq_cap, "SpMV grid exceeds the partial-sum buffer");
     hipLaunchKernelGGL((k_pcg_spmv_dir<T, ND, TL, NRP, true, D16>), dim3(*g), dim3(BS), 0, s, m,
-                       val, u, p, sv, it, seg_first, part_in, n_in, st);
+                       val, u, p, sv, it, part_in, n_in, st);
   } else {
     // grid-stride over the nodes with every block resident in one round;
     // the 8-wave variant unless holding it to 64 VGPRs spills registers,
@@ -1257,7 +1266,7 @@ static int spmv_dir_launch_d(hipStream_t s, const SpmvShape &sh, const tfm_mesh_
     static const int resident = resident_grid((const void *)kern);
     *g = grid_for(m.n_nod, BS / TL, std::min(sh.cap, resident));
     TFM_REQUIRE((int64_t)*g <= st.pq_cap, "SpMV grid exceeds the partial-sum buffer");
-    hipLaunchKernelGGL(kern, dim3(*g), dim3(BS), 0, s, m, val, u, p, sv, it, seg_first, part_in,
+    hipLaunchKernelGGL(kern, dim3(*g), dim3(BS), 0, s, m, val, u, p, sv, it, part_in,
                        n_in, st);
   }
   TFM_CHECK_LAUNCH();
@@ -1266,16 +1275,15 @@ static int spmv_dir_launch_d(hipStream_t s, const SpmvShape &sh, const tfm_mesh_
 
 template <typename T, int ND, int NRP>
 static int spmv_dir_launch(hipStream_t s, const tfm_mesh_t &m, const T *val, const T *u, T *p,
-                           T *sv, int it, int seg_first, const double *part_in, int n_in,
+                           T *sv, int it, const double *part_in, int n_in,
                            const PcgState &st, unsigned *grid_out) {
   const SpmvShape sh = spmv_shape<T>(m, NRP);
   int rc = TFM_OK;
   TFM_TEAM_SWITCH(sh.tl, {
-    rc = m.adj16 ? spmv_dir_launch_d<T, ND, TL, NRP, true>(s, sh, m, val, u, p, sv, it, seg_first,
-                                                           part_in, n_in, st, grid_out)
-                 : spmv_dir_launch_d<T, ND, TL, NRP, false>(s, sh, m, val, u, p, sv, it,
-                                                            seg_first, part_in, n_in, st,
-                                                            grid_out);
+    rc = m.adj16 ? spmv_dir_launch_d<T, ND, TL, NRP, true>(s, sh, m, val, u, p, sv, it, part_in,
+                                                           n_in, st, grid_out)
+                 : spmv_dir_launch_d<T, ND, TL, NRP, false>(s, sh, m, val, u, p, sv, it, part_in,
+                                                            n_in, st, grid_out);
   });
   return rc;
 }
@@ -1317,7 +1325,7 @@ struct tfm_pcg {
   double *part_pq = nullptr, *part_upd = nullptr, *part_aux = nullptr, *part_res = nullptr;
   PcgCoef *coef = nullptr;  // [2]: coefficients of A and B (scalar-kernel mode)
   double *res2 = nullptr;
-  int *done = nullptr, *alldone = nullptr;
+  int *done = nullptr, *alldone = nullptr, *segf = nullptr;
   int pq_cap = 0;                                          // rows of part_pq
   int res_cap = 0;                                         // rows (pairs) of part_res
   int *h_done = nullptr;                                   // pinned host mirrors
@@ -1360,6 +1368,7 @@ static int pcg_alloc(tfm_pcg *h) {
   TFM_CHECK_HIP(hipMalloc(&h->res2, sizeof(double) * 2));
   TFM_CHECK_HIP(hipMalloc(&h->done, sizeof(int) * nrp));
   TFM_CHECK_HIP(hipMalloc(&h->alldone, sizeof(int)));
+  TFM_CHECK_HIP(hipMalloc(&h->segf, sizeof(int) * nrp));
   TFM_CHECK_HIP(hipHostMalloc(&h->h_done, sizeof(int) * (nrp + 1), hipHostMallocDefault));
   TFM_CHECK_HIP(hipHostMalloc(&h->h_res, sizeof(double) * 4, hipHostMallocDefault));
   TFM_CHECK_HIP(hipHostMalloc(&h->h_scal, sizeof(double) * PCG_CHK_MAX * nrp * NSCAL,
@@ -1376,6 +1385,7 @@ static void pcg_free(tfm_pcg *h) {
   if (h->coef) (void)hipFree(h->coef);
   if (h->done) (void)hipFree(h->done);
   if (h->alldone) (void)hipFree(h->alldone);
+  if (h->segf) (void)hipFree(h->segf);
   if (h->h_done) (void)hipHostFree(h->h_done);
   if (h->h_res) (void)hipHostFree(h->h_res);
   if (h->h_scal) (void)hipHostFree(h->h_scal);
@@ -1431,8 +1441,7 @@ namespace {
 // `scal`, one-block scalar kernels prepare the coefficients of A and B.
 template <typename T> struct Ops {
   bool scal = false;
-  std::function<int(const T *, T *, T *, int, int, const double *, int, const PcgState &,
-                    unsigned *)>
+  std::function<int(const T *, T *, T *, int, const double *, int, const PcgState &, unsigned *)>
       spmv_dir;
   std::function<int(const T *, T *)> spmv;
   std::function<int(const double *, const double *, double *, double *, double *)> residual;
@@ -1462,7 +1471,7 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
   static const int xcd = env_int("TFM_XCD_MAP", 1);
   PcgState st{h->scal,     h->tol2,     h->rzold,    h->done,     h->alldone,
               h->part_pq,  h->part_upd, h->part_aux, h->pq_cap,   h->coef,
-              h->coef + 1, xcd};
+              h->coef + 1, xcd,        h->segf};
   const unsigned g1 = (unsigned)((n + 255) / 256);
   // vector kernels: one (node, rhs) per thread when a scalar kernel sums
   // their partials (at most RBW/NRP blocks, so that it sums few partial
@@ -1507,14 +1516,16 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
     TFM_CHECK_LAUNCH();
     return TFM_OK;
   };
-  // segment start: u = M r, then keep (cont) or reset (mask & ~cont) the directions
-  auto seg_start_kernels = [&](int it, int mask, int cont, const SegInfo &seg) -> int {
+  // segment start of the rhs in `mask`: u = M r, then keep (cont) or reset
+  // (mask & ~cont) their directions; `running` lists every rhs still iterating
+  auto seg_start_kernels = [&](int it, int mask, int cont, int running,
+                               const SegInfo &seg) -> int {
     TFM_CHECK_HIP(hipMemcpyAsync(h->tol2, htol.data(), sizeof(double) * NRP, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((k_pcg_seg_dots<T, ND, NRP>), dim3(gv), dim3(BS), 0, s, n_nod,
                        (const T *)rv, uv, (const T *)pv, minv, st);
     TFM_CHECK_LAUNCH();
     hipLaunchKernelGGL((k_pcg_seg_apply<T, ND, NRP>), dim3(gv), dim3(BS), 0, s, n_nod, it, mask,
-                       cont, seg, (const T *)uv, pv, sv, st, (int)gv);
+                       cont, running, seg, (const T *)uv, pv, sv, st, (int)gv);
     TFM_CHECK_LAUNCH();
     if (cont) return ops.spmv(pv, sv);
     return TFM_OK;
@@ -1531,139 +1542,151 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
     if (rc != TFM_OK) return rc;
     if (!fin[r]) mask |= 1 << r;
   }
+  for (int r = 0; r < NRP; ++r) seg.d[r] = -1;
   {
-    const int rc = seg_start_kernels(0, mask, 0, seg);
+    const int rc = seg_start_kernels(0, mask, 0, mask, seg);
     if (rc != TFM_OK) return rc;
   }
 
-  // Batches between host checks: the first of a segment has CHK iterations;
-  // later ones are sized from the contraction of rr observed so far, to end
-  // the segment with few empty launches and few host round trips.
+  // Every rhs runs its own sequence of segments: when one reaches its
+  // segment target, the host folds its correction into x and restarts it at
+  // the next batch boundary while the others keep iterating, so no rhs waits
+  // for the slowest. Batches between host checks are sized from the observed
+  // contraction of rr so that the first running rhs to reach its target
+  // stops near the batch end (few idle iterations, few host round trips).
   const int CHK = 12, CHK_MIN = 4, CHK_MAX = PCG_CHK_MAX;
-  int it = 0, n_outer = 0, seg_first = 0;
-  bool any_live = mask != 0;
-  while (any_live) {
-    bool seg_done = false;
-    int chk = CHK;
-    std::vector<double> rr_seen(NRP, -1.0);
-    std::vector<int> it_seen(NRP, -1);
-    while (!seg_done && it < h->max_iter) {
-      const int start = it;
-      const int stop = std::min(h->max_iter, it + chk);
-      for (; it < stop; ++it) {
-        if (ops.scal) {
-          hipLaunchKernelGGL((k_pcg_scal_dir<NRP>), dim3(1), dim3(BS), 0, s, it, seg_first,
-                             (const double *)h->part_upd, (int)gv, st);
-          TFM_CHECK_LAUNCH();
-        }
-        const bool timed = h->timing && it % h->timing == 0;
-        if (timed) TFM_CHECK_HIP(hipEventRecord(h->ev[2 * (it - start)], s));
-        unsigned npq = 0;
-        int rc = ops.spmv_dir(uv, pv, sv, it, seg_first, h->part_upd, (int)gv, st, &npq);
-        if (rc != TFM_OK) return rc;
-        if (timed) TFM_CHECK_HIP(hipEventRecord(h->ev[2 * (it - start) + 1], s));
-        if (ops.scal) {
-          hipLaunchKernelGGL((k_pcg_scal_alpha<NRP>), dim3(1), dim3(1024), 0, s, it,
-                             (const double *)h->part_pq, (int)npq, st);
-          TFM_CHECK_LAUNCH();
-          hipLaunchKernelGGL((k_pcg_update<T, ND, NRP, true>), dim3(gv), dim3(BSV), 0, s, n_nod, it,
-                             xv, rv, uv, (const T *)pv, (const T *)sv, minv, st,
-                             (const double *)h->part_pq, (int)npq);
-        } else {
-          hipLaunchKernelGGL((k_pcg_update<T, ND, NRP, false>), dim3(gv), dim3(BSV), 0, s, n_nod, it,
-                             xv, rv, uv, (const T *)pv, (const T *)sv, minv, st,
-                             (const double *)h->part_pq, (int)npq);
-        }
+  std::vector<int> segf(NRP, 0), nseg(NRP, 0);
+  std::vector<double> rr_seen(NRP, -1.0);
+  std::vector<int> it_seen(NRP, -1);
+  int it = 0, n_outer = 0, chk = CHK;
+  int running = mask;
+  while (running && it < h->max_iter) {
+    const int start = it;
+    const int stop = std::min(h->max_iter, it + chk);
+    for (; it < stop; ++it) {
+      if (ops.scal) {
+        hipLaunchKernelGGL((k_pcg_scal_dir<NRP>), dim3(1), dim3(BS), 0, s, it,
+                           (const double *)h->part_upd, (int)gv, st);
         TFM_CHECK_LAUNCH();
       }
-      TFM_CHECK_HIP(hipMemcpyAsync(h->h_done, h->done, sizeof(int) * NRP, hipMemcpyDeviceToHost, s));
-      TFM_CHECK_HIP(hipMemcpyAsync(h->h_scal, h->scal + (size_t)start * NRP * NSCAL,
-                                   sizeof(double) * (stop - start) * NRP * NSCAL,
-                                   hipMemcpyDeviceToHost, s));
-      TFM_CHECK_HIP(hipStreamSynchronize(s));
-      seg_done = true;
-      int end = 0;
-      for (int r = 0; r < NRP; ++r) {
-        seg_done &= h->h_done[r] >= 0;
-        end = std::max(end, h->h_done[r]);
+      const bool timed = h->timing && it % h->timing == 0;
+      if (timed) TFM_CHECK_HIP(hipEventRecord(h->ev[2 * (it - start)], s));
+      unsigned npq = 0;
+      int rc = ops.spmv_dir(uv, pv, sv, it, h->part_upd, (int)gv, st, &npq);
+      if (rc != TFM_OK) return rc;
+      if (timed) TFM_CHECK_HIP(hipEventRecord(h->ev[2 * (it - start) + 1], s));
+      if (ops.scal) {
+        hipLaunchKernelGGL((k_pcg_scal_alpha<NRP>), dim3(1), dim3(1024), 0, s, it,
+                           (const double *)h->part_pq, (int)npq, st);
+        TFM_CHECK_LAUNCH();
+        hipLaunchKernelGGL((k_pcg_update<T, ND, NRP, true>), dim3(gv), dim3(BSV), 0, s, n_nod, it,
+                           xv, rv, uv, (const T *)pv, (const T *)sv, minv, st,
+                           (const double *)h->part_pq, (int)npq);
+      } else {
+        hipLaunchKernelGGL((k_pcg_update<T, ND, NRP, false>), dim3(gv), dim3(BSV), 0, s, n_nod, it,
+                           xv, rv, uv, (const T *)pv, (const T *)sv, minv, st,
+                           (const double *)h->part_pq, (int)npq);
       }
-      // p.Kp <= 0 in an iteration that ran: K is not positive definite
-      for (int i = start; i < stop; ++i)
-        for (int r = 0; r < nrhs; ++r) {
-          if (h->h_done[r] >= 0 && i >= h->h_done[r]) continue;
-          const double eta = h->h_scal[((size_t)(i - start) * NRP + r) * NSCAL + S_PQ];
-          if (!(eta > 0.0)) {
-            set_error("the matrix is not positive definite (p.Kp <= 0 in CG)");
-            return TFM_ERR_NOCONV;
-          }
-        }
-      const double *last = h->h_scal + (size_t)(stop - 1 - start) * NRP * NSCAL;
-      // next batch: iterations until the slowest running rhs reaches its
-      // segment target at its observed rate (rr of iteration stop-1)
-      int want = 0;
+      TFM_CHECK_LAUNCH();
+    }
+    TFM_CHECK_HIP(hipMemcpyAsync(h->h_done, h->done, sizeof(int) * NRP, hipMemcpyDeviceToHost, s));
+    TFM_CHECK_HIP(hipMemcpyAsync(h->h_scal, h->scal + (size_t)start * NRP * NSCAL,
+                                 sizeof(double) * (stop - start) * NRP * NSCAL,
+                                 hipMemcpyDeviceToHost, s));
+    TFM_CHECK_HIP(hipStreamSynchronize(s));
+    // p.Kp <= 0 in an iteration that ran: K is not positive definite
+    for (int i = start; i < stop; ++i)
       for (int r = 0; r < nrhs; ++r) {
-        if (h->h_done[r] >= 0) continue;
-        const double rr = last[r * NSCAL + S_RR];
-        int k = CHK;
-        if (it_seen[r] >= 0 && rr > 0.0 && rr < rr_seen[r] && htol[r] > 0.0) {
-          const double rate = std::log(rr / rr_seen[r]) / (stop - 1 - it_seen[r]);
-          k = (int)std::ceil(std::log(htol[r] / rr) / rate) + 2;
+        if (!((running >> r) & 1) || (h->h_done[r] >= 0 && i >= h->h_done[r])) continue;
+        const double eta = h->h_scal[((size_t)(i - start) * NRP + r) * NSCAL + S_PQ];
+        if (!(eta > 0.0)) {
+          set_error("the matrix is not positive definite (p.Kp <= 0 in CG)");
+          return TFM_ERR_NOCONV;
         }
-        rr_seen[r] = rr;
-        it_seen[r] = stop - 1;
-        want = std::max(want, std::min(std::max(k, CHK_MIN), CHK_MAX));
       }
-      chk = want > 0 ? want : CHK;
-      if (h->timing) {  // launches after the last rhs stopped were empty
-        const int real_end = seg_done ? std::min(it, end) : it;
-        for (int i = start; i < real_end; ++i) {
-          if (i % h->timing) continue;
-          float ms = 0.0f;
-          TFM_CHECK_HIP(hipEventElapsedTime(&ms, h->ev[2 * (i - start)], h->ev[2 * (i - start) + 1]));
-          h->spmv_ms += ms;
-          h->spmv_count += 1;
+    if (h->timing) {  // launches after every running rhs stopped were empty
+      int end = 0;
+      bool all_stopped = true;
+      for (int r = 0; r < nrhs; ++r)
+        if ((running >> r) & 1) {
+          all_stopped &= h->h_done[r] >= 0;
+          end = std::max(end, h->h_done[r]);
         }
+      const int real_end = all_stopped ? std::min(it, end) : it;
+      for (int i = start; i < real_end; ++i) {
+        if (i % h->timing) continue;
+        float ms = 0.0f;
+        TFM_CHECK_HIP(hipEventElapsedTime(&ms, h->ev[2 * (i - start)], h->ev[2 * (i - start) + 1]));
+        h->spmv_ms += ms;
+        h->spmv_count += 1;
       }
     }
-    for (int r = 0; r < nrhs; ++r)
-      if (!fin[r]) iters[r] += std::max(0, (h->h_done[r] >= 0 ? h->h_done[r] : it) - seg_first);
-    // fold the corrections into x and replace the residuals by true ones;
-    // FP64 (whose corrections only absorb the drift of the recursive residual)
-    // and a mixed solve whose true residual stopped falling restart from the
-    // new residual, other mixed solves keep their direction
+    // the rhs that reached their segment target: fold the corrections into x
+    // and replace the residuals by true ones; FP64 (whose corrections only
+    // absorb the drift of the recursive residual) and a mixed solve whose true
+    // residual stopped falling restart from the new residual, other mixed
+    // solves keep their direction
+    const double *last = h->h_scal + (size_t)(stop - 1 - start) * NRP * NSCAL;
     int cont = 0;
     mask = 0;
-    any_live = false;
     for (int r = 0; r < NRP; ++r) seg.d[r] = h->h_done[r];
     for (int r = 0; r < nrhs; ++r) {
-      if (fin[r]) continue;
+      if (!((running >> r) & 1) || h->h_done[r] < 0) continue;
+      iters[r] += std::max(0, h->h_done[r] - segf[r]);
+      ++nseg[r];
       hipLaunchKernelGGL((k_take_column<T, NRP>), dim3(g1), dim3(256), 0, s, n, r, xv,
                          x + (int64_t)r * n);
       TFM_CHECK_LAUNCH();
       const int rc = stage(r, false);
       if (rc != TFM_OK) return rc;
-      if (!fin[r]) {
-        mask |= 1 << r;
-        const bool stuck = rnow[r] > 0.9 * rprev[r];
-        stall[r] = stuck ? stall[r] + 1 : 0;
-        if (!fp64 && !stuck && seg.d[r] > seg_first) cont |= 1 << r;
-        any_live = true;
+      rr_seen[r] = -1.0;
+      it_seen[r] = -1;
+      if (fin[r]) {
+        running &= ~(1 << r);
+        continue;
       }
+      mask |= 1 << r;
+      const bool stuck = rnow[r] > 0.9 * rprev[r];
+      stall[r] = stuck ? stall[r] + 1 : 0;
+      if (!fp64 && !stuck && seg.d[r] > segf[r]) cont |= 1 << r;
     }
     // a residual that no correction improves has reached the attainable
     // accuracy of the arithmetic: give up instead of cycling
     for (int r = 0; r < nrhs; ++r)
       if (!fin[r] && stall[r] >= 8) {
         set_error("PCG stagnated above the tolerance (attainable accuracy reached)");
-        if (outer) *outer = n_outer;
+        if (outer) *outer = *std::max_element(nseg.begin(), nseg.end());
         return TFM_ERR_NOCONV;
       }
-    ++n_outer;
-    if (!any_live || it >= h->max_iter) break;
-    const int rc = seg_start_kernels(it, mask, cont, seg);
-    if (rc != TFM_OK) return rc;
-    seg_first = it;
+    if (mask) {
+      const int rc = seg_start_kernels(it, mask, cont, running, seg);
+      if (rc != TFM_OK) return rc;
+      for (int r = 0; r < nrhs; ++r)
+        if ((mask >> r) & 1) segf[r] = it;
+    }
+    // next batch: iterations until the first running rhs reaches its segment
+    // target at its observed rate (rr of iteration stop-1); a rhs that has
+    // just started a segment counts CHK
+    int want = CHK_MAX;
+    for (int r = 0; r < nrhs; ++r) {
+      if (!((running >> r) & 1)) continue;
+      int k = CHK;
+      if (!((mask >> r) & 1)) {
+        const double rr = last[r * NSCAL + S_RR];
+        if (it_seen[r] >= 0 && rr > 0.0 && rr < rr_seen[r] && htol[r] > 0.0) {
+          const double rate = std::log(rr / rr_seen[r]) / (stop - 1 - it_seen[r]);
+          k = (int)std::ceil(std::log(htol[r] / rr) / rate) + 2;
+        }
+        rr_seen[r] = rr;
+        it_seen[r] = stop - 1;
+      }
+      want = std::min(want, std::min(std::max(k, CHK_MIN), CHK_MAX));
+    }
+    chk = want;
   }
+  for (int r = 0; r < nrhs; ++r)
+    if ((running >> r) & 1) iters[r] += std::max(0, it - segf[r]);
+  n_outer = *std::max_element(nseg.begin(), nseg.end());
   if (outer) *outer = n_outer;
   for (int r = 0; r < nrhs; ++r)
     if (!fin[r]) {
@@ -1761,9 +1784,9 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
       if (rc != TFM_OK) return rc;
       Ops<double> ops;
       ops.scal = spmv_shape<double>(m, NRP).scal;
-      ops.spmv_dir = [&](const double *u, double *p, double *sv, int it, int sf,
+      ops.spmv_dir = [&](const double *u, double *p, double *sv, int it,
                          const double *pin, int n_in, const PcgState &st, unsigned *g) -> int {
-        return spmv_dir_launch<double, ND, NRP>(s, m, sys->val64, u, p, sv, it, sf, pin, n_in, st,
+        return spmv_dir_launch<double, ND, NRP>(s, m, sys->val64, u, p, sv, it, pin, n_in, st,
                                                 g);
       };
       ops.spmv = [&](const double *xi, double *yo) -> int {
@@ -1777,9 +1800,9 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
       if (rc != TFM_OK) return rc;
       Ops<float> ops;
       ops.scal = spmv_shape<float>(m, NRP).scal;
-      ops.spmv_dir = [&](const float *u, float *p, float *sv, int it, int sf, const double *pin,
+      ops.spmv_dir = [&](const float *u, float *p, float *sv, int it, const double *pin,
                          int n_in, const PcgState &st, unsigned *g) -> int {
-        return spmv_dir_launch<float, ND, NRP>(s, m, sys->val32, u, p, sv, it, sf, pin, n_in, st,
+        return spmv_dir_launch<float, ND, NRP>(s, m, sys->val32, u, p, sv, it, pin, n_in, st,
                                                g);
       };
       ops.spmv = [&](const float *xi, float *yo) -> int {
@@ -1844,12 +1867,12 @@ extern "C" int tfm_csr_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, int64_t n,
                      (int)TFM_PC_JACOBI, (double *)h->minv);
   TFM_CHECK_LAUNCH();
   Ops<double> ops;
-  ops.spmv_dir = [&](const double *u, double *p, double *sv, int it, int sf, const double *pin,
+  ops.spmv_dir = [&](const double *u, double *p, double *sv, int it, const double *pin,
                      int n_in, const PcgState &st, unsigned *g) -> int {
     *g = grid_for(n, BS / CSR_TL, RBA);
     TFM_REQUIRE((int64_t)*g <= st.pq_cap, "SpMV grid exceeds the partial-sum buffer");
     hipLaunchKernelGGL((k_pcg_csr_dir<CSR_TL>), dim3(*g), dim3(BS), 0, s, n, rowptr, col, val, u,
-                       p, sv, it, sf, pin, n_in, st);
+                       p, sv, it, pin, n_in, st);
     TFM_CHECK_LAUNCH();
     return TFM_OK;
   };

@@ -159,6 +159,16 @@ template <class E> constexpr ShapeTable<E> make_table() {
 
 template <class E> __device__ const ShapeTable<E> &shape_table();
 
+// Logical index of this block for one-block-per-item kernels: workgroups are
+// dispatched to the 8 XCDs round-robin (block b on XCD b % 8), so the blocks
+// of XCD x take the contiguous range [x q + min(x, r), ...) of the G = 8 q + r
+// items (a bijection of [0, G)). Neighbouring items share element data,
+// nodes and vector entries, which then stay in that XCD's L2.
+__device__ __forceinline__ int64_t xcd_block() {
+  const unsigned G = gridDim.x, b = blockIdx.x, x = b & 7u, q = G >> 3, r = G & 7u;
+  return (int64_t)x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // ----------------------------------------------------------- small algebra
 // J[i][j] = d x_j / d xi_i (base.py:307-309); returns det and J^{-1}.
 template <int D> __device__ __forceinline__ double inv_jac(const double (&J)[D][D], double (&Ji)[D][D]) {

@@ -15,6 +15,7 @@ Prints one JSON line (stdout); progress goes to stderr.
 from __future__ import annotations
 
 import argparse
+import contextlib
 import json
 import os
 import sys
@@ -52,7 +53,7 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--no-reference", action="store_true",
                     help="skip the tight FP64 solve used for the accuracy report")
     ap.add_argument("--workload", default="hexa1_64",
-                    choices=["hexa1_64", "simp48", "tetra80", "hexa2_32x16"],
+                    choices=["hexa1_64", "simp48", "tetra80", "hexa2_32x16", "hyper105"],
                     help="hexa1_64: the headline Hexa1 64^3 cube (default); the others time the "
                          "SIMP, thermal and multi-load-case problems the same way (no roofline "
                          "or CPU baseline)")
@@ -368,6 +369,125 @@ def other_workload(args) -> dict:
     }
 
 
+def neo_hookean(F, params):
+    """Compressible Neo-Hookean energy mu/2 (tr C - 3) - mu ln J + lbd/2 (ln J)^2."""
+    import torch
+
+    C = F.transpose(-1, -2) @ F
+    ln_j = 0.5 * torch.logdet(C)
+    return params[0] / 2 * (torch.trace(C) - 3.0) - params[0] * ln_j + params[1] / 2 * ln_j**2
+
+
+def stretch_closed_form(mu: float, lbd: float, lam: float):
+    """Uniaxial stretch lam of a Neo-Hookean block with free lateral faces:
+    J from mu (J / lam - 1) + lbd ln J = 0, the nominal stress
+    R = mu (lam - 1 / lam) + lbd ln J / lam and dR/d(mu, lbd)."""
+    import math
+
+    J = 1.0
+    for _ in range(100):
+        J -= (mu * (J / lam - 1.0) + lbd * math.log(J)) / (mu / lam + lbd / J)
+    den = mu / lam + lbd / J
+    dj_dmu, dj_dlbd = -(J / lam - 1.0) / den, -math.log(J) / den
+    R = mu * (lam - 1.0 / lam) + lbd * math.log(J) / lam
+    return R, (lam - 1.0 / lam + lbd / (lam * J) * dj_dmu,
+               math.log(J) / lam + lbd / (lam * J) * dj_dlbd)
+
+
+def hyper_workload(args) -> dict:
+    """Finite-strain Newton path: a Neo-Hookean box of 4 x 104 x 104 cubic
+    Hexa1 cells stretched to 10x its length along x in 10 geometric
+    increments (nlgeom), then d(reaction)/d(mu, lbd) through every increment
+    (the reference's hyperelasticity benchmark problem at its largest size,
+    benchmarks/hyperelasticity.py). The homogeneous solution gives the exact
+    reaction and gradient to check against."""
+    import math
+
+    import torch
+
+    from torchfem import Solid
+    from torchfem.materials import Hyperelastic3D
+    from torchfem.mesh import cube_hexa
+
+    N, stretch, n_inc = 105, 10.0, 11
+    E, nu = 1000.0, 0.3
+    lbd, mu = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), E / (2.0 * (1.0 + nu))
+    lx = 4.0 / (N - 1)
+    nodes, elements = cube_hexa(5, N, N, lx, 1.0, 1.0)
+    params0 = torch.tensor([mu, lbd], dtype=torch.float64)
+    box = Solid(nodes.cuda(), elements.cuda(), Hyperelastic3D(neo_hookean, params0))
+    X = box.nodes
+    right = X[:, 0] == lx
+    box.constraints[X[:, 0] == 0.0, 0] = True
+    box.constraints[right, 0] = True
+    box.constraints[X[:, 1] == 0.5, 1] = True
+    box.constraints[X[:, 2] == 0.5, 2] = True
+    box.displacements[right, 0] = (stretch - 1.0) * lx
+    lam = torch.logspace(0, math.log10(stretch), n_inc, dtype=torch.float64)
+    incs = (lam - 1.0) / (stretch - 1.0)
+    R_exact, dR_exact = stretch_closed_form(mu, lbd, stretch)
+    area = 1.0  # the reaction acts on the unit x-face
+    log(f"hyperelastic stretch: {box.n_dofs} DOFs, {box.n_elem} Hexa1, {n_inc - 1} increments")
+
+    def run(precision):
+        p = params0.clone().requires_grad_(True)
+        box.material = Hyperelastic3D(neo_hookean, p).vectorize(box.n_elem)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        # the per-increment report goes to stderr with the other progress lines
+        with contextlib.redirect_stdout(sys.stderr):
+            _, f, _, _, _ = box.solve(increments=incs, nlgeom=True, differentiable_parameters=p,
+                                      precision=precision, verbose=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        reaction = f[right, 0].sum()
+        (g,) = torch.autograd.grad(reaction, p)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        info = box.last_solve_info["forward"]["increments"]
+        solvers = [q.get("solver") for inc in info for q in inc["pcg"]]
+        return {"fwd_s": t1 - t0, "bwd_s": t2 - t1,
+                "newton_iterations": [inc["newton_iterations"] for inc in info],
+                "linear_solves": {k: solvers.count(k) for k in set(solvers)},
+                "reaction_rel_err_vs_closed_form": abs(float(reaction) - R_exact * area) / R_exact,
+                "grad_rel_err_vs_closed_form": float(
+                    (g - torch.tensor(dR_exact, dtype=torch.float64)).norm()
+                    / torch.tensor(dR_exact).norm())}
+
+    for i in range(args.warmup):
+        run("mixed")
+        log(f"hyper: warmup {i + 1}/{args.warmup} done")
+    res = {}
+    for prec in ("mixed", "fp64") if args.fp64_steps > 0 else ("mixed",):
+        runs = [run(prec) for _ in range(max(1, args.steps if prec == "mixed" else args.fp64_steps))]
+        log(f"hyper {prec}: {[round(r['fwd_s'] + r['bwd_s'], 3) for r in runs]} s")
+        best = min(runs, key=lambda r: r["fwd_s"] + r["bwd_s"])
+        res[prec] = dict(best, fwd_s_all=[r["fwd_s"] for r in runs],
+                         bwd_s_all=[r["bwd_s"] for r in runs])
+    m = res["mixed"]
+    total = m["fwd_s"] + m["bwd_s"]
+    return {
+        "metric": "Neo-Hookean stretch (10x, 10 increments, nlgeom) solve + adjoint "
+                  "d(reaction)/d(mu, lbd) time & DOFs/s",
+        "value": box.n_dofs / total,
+        "unit": "DOFs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total * 1e3,
+        "higher_is_better": True,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "hyper105", "N": N, "n_dofs": box.n_dofs, "n_elem": box.n_elem,
+                   "increments": n_inc - 1, "stretch": stretch,
+                   "newton": "rtol 1e-8, atol 1e-6, PCG stol 1e-10 (solve defaults)"},
+        "results": res,
+        "reference_published": {
+            "source": "benchmarks/results/hyperelasticity_rtx5090_cuda.json (N=105)",
+            "hardware": "RTX 5090, AmgX, FP64", "fwd_s": 39.6331, "bwd_s": 7.5877},
+    }
+
+
 def main() -> None:
     args = parse()
     if args.gpus != 1:
@@ -382,6 +502,9 @@ def main() -> None:
     torch.set_default_dtype(torch.float64)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.workload == "hyper105":
+        print(json.dumps(hyper_workload(args)), flush=True)
         return
     if args.workload != "hexa1_64":
         print(json.dumps(other_workload(args)), flush=True)

@@ -148,8 +148,9 @@ class FiniteStrainRun:
     """State of one finite-strain solve: the model, its operators, the
     constraint mask, the solver settings and the last assembled tangent."""
 
-    def __init__(self, fem, cfg: dict, nlgeom: bool):
+    def __init__(self, fem, cfg: dict, nlgeom: bool, verbose: bool = False):
         self.fem = fem
+        self.verbose = verbose
         self.mat = fem.material
         self.kin = Kinematics(fem)
         self.con = fem.constraints.reshape(-1).to(fem.device)
@@ -235,6 +236,10 @@ class FiniteStrainRun:
                 break
             d, info = self.solve_linear(self.system, res)
             pcg.append(info)
+            if self.verbose:
+                its = info["iterations"]
+                print(f"  newton {i}: |R| {rn:.3e}, {info['solver']} "
+                      f"{its[0] if isinstance(its, list) else its} iterations", flush=True)
             du = du - d
         raise RuntimeError("Newton-Raphson iteration did not converge. Residual norms: "
                            + ", ".join(f"{r:.3e}" for r in hist[-4:]))
@@ -283,7 +288,7 @@ def solve_finite(fem, incs: list[float], cfg: dict, nlgeom: bool, params: tuple,
     substep runs NewtonStep from the previous substep's increment, then
     evaluates the converged state; results are stored at the requested
     increments."""
-    run = FiniteStrainRun(fem, cfg, nlgeom)
+    run = FiniteStrainRun(fem, cfg, nlgeom, verbose)
     kin = run.kin
     dev = fem.device
     dt = torch.float64
@@ -352,7 +357,7 @@ def solve_finite(fem, incs: list[float], cfg: dict, nlgeom: bool, params: tuple,
                 last = run.log[-1] if run.log else {}
                 print(f"increment {n}: load factor {lam:.6g} | newton "
                       f"{last.get('newton_iterations')} | res {last.get('res', float('nan')):.3e}"
-                      f" | cutbacks {cutbacks}")
+                      f" | cutbacks {cutbacks} | {time.perf_counter() - t0:.2f} s", flush=True)
 
     u_all = torch.stack(out_u).reshape(N, -1, nd)
     f_all = torch.stack(out_f).reshape(N, -1, nd)

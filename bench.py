@@ -119,7 +119,7 @@ def measured_traffic():
     """HBM bytes per launch of the FP32 SpMV kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a copy of known size);
     None when that summary is absent."""
-    path = ROOT / "profiles" / "spmv_dir_f32_hexa1_64_hbm_traffic.md"
+    path = ROOT / "profiles" / "pcg_hexa1_64.md"
     try:
         for line in path.read_text().splitlines():
             if line.startswith("traffic_bytes_per_launch:"):
@@ -563,7 +563,9 @@ def main() -> None:
     avg_s = spmv_ms / max(spmv_n, 1) / 1e3
     achieved = bytes_launch / avg_s / 1e9 if spmv_n else None
     roof = {
-        "kernel": "k_pcg_spmv_dir<float,3,16,1,false,true>",
+        # the launcher runs the 6-wave variant of this kernel on gfx950 (the
+        # 8-wave one would spill); the committed kernel trace names it
+        "kernel": "k_pcg_spmv_dir_w6<float,3,16,1,false,true>",
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,

@@ -31,6 +31,7 @@ __all__ = [
     "build_pattern", "iso3d_C", "plane_stress_C", "iso_KAPPA",
     "OracleModel", "simp_compliance_sensitivity", "hexa_nnz_closed_form",
     "neo_hookean_psi", "hyper_step", "oracle_integrate_hyper", "oracle_solve_hyper",
+    "oracle_integrate_incremental", "oracle_solve_incremental", "hyper_material", "j2_step",
 ]
 
 
@@ -793,51 +794,123 @@ def hyper_step(psi, F: Tensor, params: Tensor):
     return P.reshape(*lead, 3, 3), A.reshape(*lead, 3, 3, 3, 3)
 
 
-def oracle_integrate_hyper(model: "OracleModel", psi, params: Tensor, grad_prev: Tensor,
-                           du: Tensor, need_k: bool = True):
-    """Mechanics.integrate_material for a hyperelastic material
-    (base.py:1036-1092): per Gauss point H = du B^T, F = grad_prev + H,
-    (P, A) = step, f += w detJ B^T P, k += w detJ B^T A B. params [E, p]."""
+def oracle_integrate_incremental(model: "OracleModel", step, grad_prev: Tensor, flux_prev: Tensor,
+                                 state_prev: Tensor, du: Tensor, need_k: bool = True):
+    """Mechanics.integrate_material for an incremental material
+    (base.py:1036-1092): per Gauss point H = du B^T, (stress, state, A) =
+    step(H, grad_prev, flux_prev, state_prev), f += w detJ B^T stress,
+    k += w detJ B^T A B, F = grad_prev + H. Returns k, f, F, stress, state."""
     _, B, detJ = model.eval_shape_functions(model.etype.ipoints)
     E, npe = model.n_elem, model.etype.nodes
     due = du.view(-1, 3)[model.elements].reshape(E, -1, 3).transpose(-1, -2)
     f = torch.zeros(E, 3 * npe, dtype=DT)
     k = torch.zeros(E, 3 * npe, 3 * npe, dtype=DT) if need_k else None
-    Fs, Ps = [], []
+    Fs, Ps, Ss = [], [], []
     for i, w in enumerate(model.etype.iweights):
         H = due @ B[i].transpose(-1, -2)
-        F = grad_prev[i] + H
-        P, A = hyper_step(psi, F, params)
-        Fs.append(F)
+        P, state, A = step(H, grad_prev[i], flux_prev[i], state_prev[i])
+        Fs.append(grad_prev[i] + H)
         Ps.append(P)
+        Ss.append(state)
         fc = torch.einsum("..., ...iI,...Ai->...IA", detJ[i], B[i], P)
         f = f + float(w) * fc.reshape(-1, 3 * npe)
         if need_k:
             BCB = torch.einsum("...Jp,...iJkL,...Lq->...piqk", B[i], A, B[i])
             k = k + torch.einsum("j,jkl->jkl", detJ[i], BCB.reshape(-1, 3 * npe, 3 * npe)) * float(w)
-    return k, f, torch.stack(Fs), torch.stack(Ps)
+    return k, f, torch.stack(Fs), torch.stack(Ps), torch.stack(Ss)
 
 
-def oracle_solve_hyper(model: "OracleModel", psi, params: Tensor, constraints: Tensor,
-                       forces: Tensor, displacements: Tensor, increments, nlgeom: bool = True,
-                       rtol: float = 1e-10, atol: float = 1e-10, max_iter: int = 30):
-    """FEM.solve's incremental Newton loop (base.py:700-880) for a
-    hyperelastic material, without cutbacks: per increment n the loads are
-    lam_n F_ext and (lam_n - lam_{n-1}) U_D, Newton starts from the previous
-    increment (du = du_{n-1}), K is reassembled every iteration with nlgeom
-    and kept from the first evaluation without it (base.py:1031-1033), and
+def hyper_material(psi, params: Tensor):
+    """Hyperelastic3D.step (hyperelasticity.py:83-127) as an oracle step."""
+    def step(H, F_prev, stress, state):
+        P, A = hyper_step(psi, F_prev + H, params)
+        return P, state, A
+    return step
+
+
+def oracle_integrate_hyper(model: "OracleModel", psi, params: Tensor, grad_prev: Tensor,
+                           du: Tensor, need_k: bool = True):
+    """oracle_integrate_incremental for a hyperelastic material; params [E, p].
+    Returns k, f, F, P."""
+    n_gp = len(model.etype.iweights)
+    zero = torch.zeros(n_gp, model.n_elem, 3, 3, dtype=DT)
+    k, f, F, P, _ = oracle_integrate_incremental(
+        model, hyper_material(psi, params), grad_prev, zero,
+        torch.zeros(n_gp, model.n_elem, 0, dtype=DT), du, need_k)
+    return k, f, F, P
+
+
+def j2_step(E: float, nu: float, sigma_f, sigma_f_prime, tol: float = 1e-5, max_iter: int = 10):
+    """IsotropicPlasticity3D.step (plasticity.py:67-215), restated point by
+    point: trial stress sigma_n + C : sym(H), von Mises check
+    |dev| > sqrt(2/3) sigma_f(q), the scalar Newton for the plastic
+    multiplier dg on |dev| - 2G dg - sqrt(2/3) sigma_f(q + sqrt(2/3) dg) = 0,
+    the return sigma = trial - 2G dg n and the consistent tangent
+    C - 2G/(1 + sigma_f'/(3G)) n(x)n - 4G^2 dg/|dev| (I_dev - n(x)n)."""
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    G = E / (2.0 * (1.0 + nu))
+    r23 = math.sqrt(2.0 / 3.0)
+    I = torch.eye(3, dtype=DT)
+    I4 = torch.einsum("ij,kl->ijkl", I, I)
+    I4s = 0.5 * (torch.einsum("ik,jl->ijkl", I, I) + torch.einsum("il,jk->ijkl", I, I))
+    C = lam * I4 + 2.0 * G * I4s
+
+    def point(H, stress, q):
+        eps = 0.5 * (H + H.T)
+        trial = stress + torch.einsum("ijkl,kl->ij", C, eps)
+        dev = trial - torch.trace(trial) / 3.0 * I
+        nrm = float(torch.linalg.norm(dev))
+        if nrm - r23 * float(sigma_f(torch.tensor(q, dtype=DT))) <= 0.0:
+            return trial, q, C.clone()
+        dg, qn = 0.0, q
+        for _ in range(max_iter):
+            res = nrm - 2.0 * G * dg - r23 * float(sigma_f(torch.tensor(qn, dtype=DT)))
+            dg += res / (2.0 * G + 2.0 / 3.0 * float(sigma_f_prime(torch.tensor(qn, dtype=DT))))
+            qn = q + r23 * dg
+            if abs(res) < tol:
+                break
+        n = dev / nrm
+        nn = torch.einsum("ij,kl->ijkl", n, n)
+        a = 2.0 * G / (1.0 + float(sigma_f_prime(torch.tensor(qn, dtype=DT))) / (3.0 * G))
+        b = 4.0 * G * G * dg / nrm
+        return trial - 2.0 * G * dg * n, qn, C - a * nn - b * (I4s - I4 / 3.0 - nn)
+
+    def step(H, F_prev, stress, state):
+        lead = H.shape[:-2]
+        Hf, Sf, qf = H.reshape(-1, 3, 3), stress.reshape(-1, 3, 3), state.reshape(-1)
+        out = [point(Hf[t], Sf[t], float(qf[t])) for t in range(Hf.shape[0])]
+        P = torch.stack([o[0] for o in out]).reshape(*lead, 3, 3)
+        q = torch.tensor([o[1] for o in out], dtype=DT).reshape(*lead, 1)
+        A = torch.stack([o[2] for o in out]).reshape(*lead, 3, 3, 3, 3)
+        return P, q, A
+    return step
+
+
+def oracle_solve_incremental(model: "OracleModel", step, n_state: int, constraints: Tensor,
+                             forces: Tensor, displacements: Tensor, increments,
+                             nlgeom: bool = True, rtol: float = 1e-10, atol: float = 1e-10,
+                             max_iter: int = 30):
+    """FEM.solve's incremental Newton loop (base.py:700-880) without cutbacks:
+    per increment n the loads are lam_n F_ext and (lam_n - lam_{n-1}) U_D,
+    Newton starts from the previous increment (du = du_{n-1}), K is
+    reassembled at every evaluation with nlgeom or internal state and kept
+    from the first evaluation otherwise (base.py:1031-1033), and
     du -= K^{-1} R until |R| < rtol |R_0| or |R| < atol (sparse.py:572-623).
-    Returns the final u, f (internal forces incl. reactions), the Gauss-point
-    mean of the stress (Cauchy F P / J with nlgeom, else P) and of F."""
+    The stored flux is the stress the step returns (Cauchy F P / J with
+    nlgeom) and is passed to the next increment's step. Returns the final u,
+    f (internal forces incl. reactions), the Gauss-point means of the flux,
+    of F and of the state."""
     con = torch.nonzero(constraints.ravel(), as_tuple=False).ravel()
     n = model.n_dofs
-    grad = torch.eye(3, dtype=DT).expand(len(model.etype.iweights), model.n_elem, 3, 3).clone()
+    n_gp = len(model.etype.iweights)
+    grad = torch.eye(3, dtype=DT).expand(n_gp, model.n_elem, 3, 3).clone()
+    flux = torch.zeros(n_gp, model.n_elem, 3, 3, dtype=DT)
+    state = torch.zeros(n_gp, model.n_elem, n_state, dtype=DT)
     u = torch.zeros(n, dtype=DT)
     du = torch.zeros(n, dtype=DT)
     K = None
     lam_prev = float(increments[0])
     f_int = torch.zeros(n, dtype=DT)
-    P_last = None
     for lam_n in [float(v) for v in increments[1:]]:
         F_ext = lam_n * forces.ravel()
         DU = (lam_n - lam_prev) * displacements.ravel()
@@ -845,8 +918,9 @@ def oracle_solve_hyper(model: "OracleModel", psi, params: Tensor, constraints: T
         for it in range(max_iter):
             du_bc = du.clone()
             du_bc[con] = DU[con]
-            need_k = K is None or nlgeom
-            k, fe, _, _ = oracle_integrate_hyper(model, psi, params, grad, du_bc, need_k)
+            need_k = K is None or nlgeom or n_state > 0
+            k, fe, _, _, _ = oracle_integrate_incremental(model, step, grad, flux, state, du_bc,
+                                                          need_k)
             if need_k:
                 K = model.csr(model.assemble_matrix(k, con))
             R = model.assemble_rhs(fe) - F_ext
@@ -859,13 +933,21 @@ def oracle_solve_hyper(model: "OracleModel", psi, params: Tensor, constraints: T
         else:
             raise RuntimeError("Newton-Raphson iteration did not converge.")
         du[con] = DU[con]
-        _, fe, grad, P_last = oracle_integrate_hyper(model, psi, params, grad, du, need_k=False)
+        _, fe, grad, P, state = oracle_integrate_incremental(model, step, grad, flux, state, du,
+                                                             need_k=False)
+        flux = grad @ P / torch.linalg.det(grad)[..., None, None] if nlgeom else P
         f_int = model.assemble_rhs(fe)
         u = u + du
         lam_prev = lam_n
-    if nlgeom:
-        J = torch.linalg.det(grad)[..., None, None]
-        flux = grad @ P_last / J
-    else:
-        flux = P_last
-    return u.reshape(-1, 3), f_int.reshape(-1, 3), flux.mean(0), grad.mean(0)
+    return u.reshape(-1, 3), f_int.reshape(-1, 3), flux.mean(0), grad.mean(0), state.mean(0)
+
+
+def oracle_solve_hyper(model: "OracleModel", psi, params: Tensor, constraints: Tensor,
+                       forces: Tensor, displacements: Tensor, increments, nlgeom: bool = True,
+                       rtol: float = 1e-10, atol: float = 1e-10, max_iter: int = 30):
+    """oracle_solve_incremental for a hyperelastic material; params [E, p].
+    Returns u, f, mean flux, mean F."""
+    u, f, flux, grad, _ = oracle_solve_incremental(
+        model, hyper_material(psi, params), 0, constraints, forces, displacements, increments,
+        nlgeom, rtol, atol, max_iter)
+    return u, f, flux, grad

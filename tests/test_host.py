@@ -132,3 +132,31 @@ def test_hyperelastic_material_matches_oracle():
     # P is linear in (mu, lam): dP/dparams . params == P
     (gp,) = torch.autograd.grad(mat.stress(F).sum(), base)
     assert abs(float(gp @ base.detach()) - float(P_o.sum())) < 1e-9 * float(P_o.abs().sum())
+
+
+def test_plastic_material_matches_oracle():
+    """IsotropicPlasticity3D.step (the host side of the incremental path)
+    against the oracle's point-by-point J2 restatement, from a pre-stressed,
+    pre-strained state, batched over [points, elements] with a nonlinear
+    hardening law."""
+    from torchfem.materials import IsotropicPlasticity3D
+
+    def sf(q):
+        return 200.0 + 150.0 * (1.0 - torch.exp(-20.0 * q))
+
+    def sfp(q):
+        return 3000.0 * torch.exp(-20.0 * q)
+
+    g = torch.Generator().manual_seed(3)
+    mag = torch.logspace(-5, -0.5, 5, dtype=torch.float64)[None, :, None, None]
+    H = mag * torch.randn(2, 5, 3, 3, generator=g, dtype=torch.float64)
+    S = 20.0 * torch.randn(2, 5, 3, 3, generator=g, dtype=torch.float64)
+    S = 0.5 * (S + S.transpose(-1, -2))
+    q = 0.02 * torch.rand(2, 5, 1, generator=g, dtype=torch.float64)
+    mat = IsotropicPlasticity3D(1000.0, 0.3, sf, sfp, tolerance=1e-12, max_iter=50).vectorize(5)
+    P, st, A = mat.step(H, None, S, q, None, None, 0)
+    P_o, st_o, A_o = O.j2_step(1000.0, 0.3, sf, sfp, tol=1e-12, max_iter=50)(H, None, S, q)
+    assert bool((st > q).any()) and bool((st == q).any())  # some points yield, some not
+    assert torch.allclose(P, P_o, rtol=1e-11, atol=1e-9)
+    assert torch.allclose(st, st_o, rtol=1e-11, atol=1e-14)
+    assert torch.allclose(A, A_o, rtol=1e-10, atol=1e-8)

@@ -536,3 +536,123 @@ def test_hyperelastic_uniaxial_stretch_reaction():
                                       (lam_pts - 1.0) / U, nlgeom=True)
     R, _ = uniaxial_stretch_closed_form(mu, lbd, 1.0 + U)
     assert abs(float(f[right, 0].sum()) - R) < 1e-8 * R
+
+
+# --------------------------------------------------------------- plasticity
+def _sf_lin(q):
+    return 200.0 + 50.0 * q
+
+
+def _sfp_lin(q):
+    return 50.0 + 0.0 * q
+
+
+def _sf_sat(q):  # saturating (nonlinear) hardening
+    return 200.0 + 150.0 * (1.0 - torch.exp(-20.0 * q))
+
+
+def _sfp_sat(q):
+    return 3000.0 * torch.exp(-20.0 * q)
+
+
+def test_j2_step_elastic_and_return_mapping():
+    """The restated J2 step: a small strain stays elastic (stress C : eps,
+    state unchanged, the reference's test_materials.py:229-246); a large
+    deviatoric strain returns to the yield surface, |dev sigma| =
+    sqrt(2/3) sigma_f(q), along the trial direction, with q > 0; the
+    hydrostatic part stays elastic (3 K tr eps / 3)."""
+    E, nu = 210e3, 0.3
+    step = O.j2_step(E, nu, _sf_lin, _sfp_lin, tol=1e-10)
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    G = E / (2 * (1 + nu))
+    H = torch.zeros(2, 3, 3, dtype=DT)
+    H[0] = 1e-6 * torch.eye(3, dtype=DT)
+    H[1] = torch.tensor([[0.01, 0.002, 0.0], [0.0, -0.004, 0.001], [0.0, 0.0, -0.003]], dtype=DT)
+    P, q, A = step(H, torch.eye(3, dtype=DT).expand(2, 3, 3), torch.zeros(2, 3, 3, dtype=DT),
+                   torch.zeros(2, 1, dtype=DT))
+    eps = 0.5 * (H + H.transpose(-1, -2))
+    C = O.iso3d_C(torch.tensor([E], dtype=DT), torch.tensor(nu, dtype=DT))[0]
+    assert torch.allclose(P[0], torch.einsum("ijkl,kl->ij", C, eps[0]), rtol=1e-12)
+    assert float(q[0]) == 0.0
+    I = torch.eye(3, dtype=DT)
+    dev = P[1] - torch.trace(P[1]) / 3 * I
+    assert abs(float(dev.norm()) - math.sqrt(2 / 3) * float(_sf_lin(q[1]))) < 1e-8
+    trial = torch.einsum("ijkl,kl->ij", C, eps[1])
+    dev_t = trial - torch.trace(trial) / 3 * I
+    assert torch.allclose(dev / dev.norm(), dev_t / dev_t.norm(), atol=1e-12)
+    assert float(q[1]) > 0.0
+    K = lam + 2 * G / 3
+    assert abs(float(torch.trace(P[1])) - 3 * K * float(torch.trace(eps[1]))) < 1e-6
+
+
+@pytest.mark.parametrize("hard", ["linear", "saturating"])
+def test_j2_consistent_tangent_matches_differences(hard):
+    """The restated consistent tangent against central differences of the
+    step's stress in the strain, from a pre-stressed, pre-strained state, at
+    an elastic and at a yielding point."""
+    sf, sfp = (_sf_lin, _sfp_lin) if hard == "linear" else (_sf_sat, _sfp_sat)
+    step = O.j2_step(1000.0, 0.3, sf, sfp, tol=1e-12, max_iter=50)
+    base = torch.tensor([[0.1, 0.03, 0.0], [0.02, -0.05, 0.01], [0.0, 0.0, -0.04]], dtype=DT)
+    S0 = torch.tensor([[50.0, 10.0, 0.0], [10.0, -20.0, 5.0], [0.0, 5.0, 0.0]], dtype=DT)
+    q0 = torch.tensor([[0.01]], dtype=DT)
+    for scale in (0.01, 5.0):  # elastic, plastic
+        H = scale * base
+        P, q, A = step(H[None], torch.eye(3, dtype=DT)[None], S0[None], q0)
+        assert (float(q[0, 0]) > 0.01) == (scale > 1.0)
+        h = 1e-7
+        for k in range(3):
+            for l in range(3):
+                d = torch.zeros(3, 3, dtype=DT)
+                d[k, l] = h
+                Pp, _, _ = step((H + d)[None], torch.eye(3, dtype=DT)[None], S0[None], q0)
+                Pm, _, _ = step((H - d)[None], torch.eye(3, dtype=DT)[None], S0[None], q0)
+                fd = (Pp - Pm)[0] / (2 * h)
+                # A is the derivative w.r.t. the symmetric strain: dP/dH_kl
+                # = (A[..., k, l] + A[..., l, k]) / 2
+                an = 0.5 * (A[0, :, :, k, l] + A[0, :, :, l, k])
+                assert torch.allclose(fd, an, rtol=1e-5, atol=1e-6 * float(A.abs().max()))
+
+
+def uniaxial_plastic_closed_form(E, nu, s0, Hh, strain):
+    """Uniaxial stress in a J2 solid with sigma_f(q) = s0 + Hh q: sigma =
+    E strain below s0 / E, else (s0 + Hh strain) / (1 + Hh / E); the
+    lateral strain -nu sigma / E - eps_p / 2 (plastic incompressibility)."""
+    if E * strain <= s0:
+        sig, ep = E * strain, 0.0
+    else:
+        sig = (s0 + Hh * strain) / (1.0 + Hh / E)
+        ep = (sig - s0) / Hh
+    return sig, -nu * sig / E - 0.5 * ep, ep
+
+
+def _plastic_box(n, s0, Hh, strain):
+    nodes, el = O.cube_hexa(n, n, n)
+    m = O.OracleModel(nodes, el, O.Hexa1, "solid", O.iso3d_C(torch.full((len(el),), 1.0), 0.3))
+    con = torch.zeros(len(nodes), 3, dtype=torch.bool)
+    con[nodes[:, 0] == 0.0, 0] = True
+    con[nodes[:, 1] == 0.0, 1] = True
+    con[nodes[:, 2] == 0.0, 2] = True
+    right = nodes[:, 0] == 1.0
+    con[right, 0] = True
+    D = torch.zeros(len(nodes), 3, dtype=DT)
+    D[right, 0] = strain
+    return nodes, m, con, D, right
+
+
+def test_j2_uniaxial_tension_closed_form():
+    """A J2 cube with linear hardening pulled to three times its yield
+    strain in four increments (roller supports on three faces): the
+    reaction is the closed-form uniaxial stress, the lateral contraction its
+    elastic plus plastically incompressible part, q the plastic strain."""
+    E, nu, s0, Hh = 1000.0, 0.3, 2.0, 100.0
+    strain = 3.0 * s0 / E
+    nodes, m, con, D, right = _plastic_box(3, s0, Hh, strain)
+    step = O.j2_step(E, nu, lambda q: s0 + Hh * q, lambda q: Hh + 0.0 * q, tol=1e-12)
+    u, f, flux, grad, state = O.oracle_solve_incremental(
+        m, step, 1, con, torch.zeros_like(D), D, torch.tensor([0.0, 0.25, 0.5, 0.75, 1.0]),
+        nlgeom=False, rtol=1e-12, atol=1e-12)
+    sig, lat, ep = uniaxial_plastic_closed_form(E, nu, s0, Hh, strain)
+    assert abs(float(f[right, 0].sum()) - sig) < 1e-9 * sig
+    assert torch.allclose(u[:, 1], lat * nodes[:, 1], atol=1e-12)
+    assert torch.allclose(flux[:, 0, 0], torch.full_like(flux[:, 0, 0], sig), rtol=1e-9)
+    assert torch.allclose(state[:, 0], torch.full_like(state[:, 0], ep), rtol=1e-9)

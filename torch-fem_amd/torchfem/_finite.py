@@ -1,4 +1,4 @@
-"""Finite-strain path of FEM.solve for hyperelastic materials.
+"""Incremental Newton path of FEM.solve for hyperelastic and plastic materials.
 
 The reference solves every model by load increments with a Newton loop whose
 residual is integrate_material + assemble_rhs and whose tangent is
@@ -160,6 +160,7 @@ class FiniteStrainRun:
         self.log: list[dict] = []
         self._krylov = None
         self._csr = None
+        self.last_iter = 0
 
     def solve_linear(self, system: TangentSystem, rhs: Tensor,
                      adjoint: bool = False) -> tuple[Tensor, dict]:
@@ -195,19 +196,38 @@ class FiniteStrainRun:
             self._krylov.close()
             self._krylov = None
 
-    def residual(self, du: Tensor, grad_prev: Tensor, F_ext: Tensor, DU: Tensor) -> Tensor:
+    def update(self, H: Tensor, grad_prev: Tensor, flux_prev: Tensor, state_prev: Tensor,
+               it: int, tangent: bool = True, mat=None):
+        """Material update at every Gauss point from the increment H = grad du:
+        (stress, state, tangent or None). A hyperelastic material evaluates
+        P(F) and dP/dF at F = grad_prev + H; other incremental materials run
+        their step from (grad_prev, flux_prev, state_prev) (materials/base.py:48-91)."""
+        mat = self.mat if mat is None else mat
+        if getattr(mat, "finite_strain", False):
+            F = grad_prev + H
+            if tangent:
+                P, A = mat.stress_tangent(F)
+                return P, state_prev, A
+            return mat.stress(F), state_prev, None
+        return mat.step(H, grad_prev, flux_prev, state_prev, None, None, it)
+
+    def residual(self, du: Tensor, grad_prev: Tensor, flux_prev: Tensor, state_prev: Tensor,
+                 F_ext: Tensor, DU: Tensor, it: int = 0, mat=None) -> Tensor:
         """Differentiable residual F_int(du with du[con] = DU[con]) - F_ext,
         zero on constrained DOFs (the eval_residual of base.py:707-740)."""
         du_bc = torch.where(self.con, DU, du)
-        F = grad_prev + _GradOp.apply(self.kin, du_bc)
-        F_int = _ForceOp.apply(self.kin, self.mat.stress(F))
+        H = _GradOp.apply(self.kin, du_bc)
+        P, _, _ = self.update(H, grad_prev, flux_prev, state_prev, it, tangent=False, mat=mat)
+        F_int = _ForceOp.apply(self.kin, P)
         return torch.where(self.con, torch.zeros_like(F_int), F_int - F_ext)
 
-    def newton(self, du: Tensor, grad_prev: Tensor, F_ext: Tensor, DU: Tensor) -> Tensor:
+    def newton(self, du: Tensor, grad_prev: Tensor, flux_prev: Tensor, state_prev: Tensor,
+               F_ext: Tensor, DU: Tensor) -> Tensor:
         """Newton iterations du -= K^{-1} R until |R| < rtol |R_0| or |R| < atol
         (NewtonRaphsonAdjoint.forward, sparse.py:572-623). The tangent is
-        reassembled at every evaluation with nlgeom and kept from the first
-        evaluation of the solve without it (base.py:1031-1035)."""
+        reassembled at every evaluation with nlgeom or an internal state and
+        kept from the first evaluation of the solve otherwise
+        (base.py:1031-1035)."""
         cfg, kin, fem = self.cfg, self.kin, self.fem
         t0 = time.perf_counter()
         du = du.detach().clone()
@@ -217,12 +237,12 @@ class FiniteStrainRun:
         hist = []
         for i in range(cfg["max_iter"]):
             du_bc = torch.where(self.con, DU, du)
-            F = grad_prev + kin.grad(du_bc)
-            if self.system is None or self.nlgeom:
-                P, A = self.mat.stress_tangent(F)
+            need_k = self.system is None or self.nlgeom or self.mat.n_state > 0
+            P, _, A = self.update(kin.grad(du_bc), grad_prev, flux_prev, state_prev, i,
+                                  tangent=need_k)
+            if need_k:
                 self.system = TangentSystem(fem, kin.matrices(A), cfg["precision"])
-            else:
-                P = self.mat.stress(F)
+            self.last_iter = i
             res = torch.where(self.con, zero, kin.force(P) - F_ext)
             rn = float(torch.linalg.norm(res))
             hist.append(rn)
@@ -246,43 +266,47 @@ class FiniteStrainRun:
 
 
 class NewtonStep(torch.autograd.Function):
-    """One converged increment du = Newton(du0; grad_prev, F_ext, DU, params)
+    """One converged increment
+    du = Newton(du0; grad_prev, flux_prev, state_prev, F_ext, DU, material)
     with the implicit adjoint of NewtonRaphsonAdjoint.backward
-    (sparse.py:672-724): K lambda = dL/ddu on the PCG (K is symmetric), then
-    -lambda pulled back through the recomputed residual to the previous
-    deformation gradient, the loads and the parameters."""
+    (sparse.py:672-724): K lambda = dL/ddu with the last assembled tangent,
+    then -lambda pulled back through the recomputed residual to the previous
+    deformation gradient, stress and state, the loads and the tensors the
+    material's update reads (its `tensors()`). Those tensors are inputs of
+    this node, so autograd carries their gradients on to whatever they were
+    computed from (the user's differentiable parameters); the pull-back
+    itself runs on detached copies and never re-enters the caller's graph."""
 
     @staticmethod
-    def forward(ctx, run: FiniteStrainRun, du0, grad_prev, F_ext, DU, *params):
-        du = run.newton(du0, grad_prev, F_ext, DU)
+    def forward(ctx, run: FiniteStrainRun, names: tuple, du0, grad_prev, flux_prev, state_prev,
+                F_ext, DU, *mat_tensors):
+        du = run.newton(du0, grad_prev, flux_prev, state_prev, F_ext, DU)
         ctx.run = run
+        ctx.names = names
         ctx.system = run.system
-        ctx.save_for_backward(du, grad_prev, F_ext, DU, *params)
+        ctx.it = run.last_iter
+        ctx.save_for_backward(du, grad_prev, flux_prev, state_prev, F_ext, DU, *mat_tensors)
         return du
 
     @staticmethod
     def backward(ctx, gdu):
         run = ctx.run
-        du, grad_prev, F_ext, DU, *params = ctx.saved_tensors
+        du, *prev = ctx.saved_tensors[:6]
+        mt = ctx.saved_tensors[6:]
         lam, _ = run.solve_linear(ctx.system, gdu, adjoint=True)
         needs = ctx.needs_input_grad
         with torch.enable_grad():
-            locs = [t.detach().requires_grad_(True) for t in (grad_prev, F_ext, DU)]
-            r = run.residual(du.detach(), *locs)
-            targets = locs + [p for p in params if p.requires_grad]
-            grads = list(torch.autograd.grad(r, targets, grad_outputs=-lam, allow_unused=True,
-                                             retain_graph=True))
-        g_state = grads[:3]
-        g_par = iter(grads[3:])
-        g_params = [next(g_par) if p.requires_grad else None for p in params]
-        out = [None, None]
-        for k, g in enumerate(g_state):
-            out.append(g if needs[2 + k] else None)
-        return (*out, *g_params)
+            locs = [t.detach().requires_grad_(True) for t in prev]
+            mloc = [t.detach().requires_grad_(True) for t in mt]
+            mat = run.mat.replace(**dict(zip(ctx.names, mloc)))
+            r = run.residual(du.detach(), *locs, it=ctx.it, mat=mat)
+            grads = torch.autograd.grad(r, locs + mloc, grad_outputs=-lam, allow_unused=True)
+        return (None, None, None,
+                *[g if needs[3 + k] else None for k, g in enumerate(grads)])
 
 
-def solve_finite(fem, incs: list[float], cfg: dict, nlgeom: bool, params: tuple,
-                 track: bool, cutback_factor: float, growth_factor: float, max_cutbacks: int,
+def solve_finite(fem, incs: list[float], cfg: dict, nlgeom: bool, track: bool,
+                 cutback_factor: float, growth_factor: float, max_cutbacks: int,
                  aggregate: bool, verbose: bool):
     """Incremental loading with automatic cutback (base.py:742-880): every
     substep runs NewtonStep from the previous substep's increment, then
@@ -305,9 +329,12 @@ def solve_finite(fem, incs: list[float], cfg: dict, nlgeom: bool, params: tuple,
     f_cur = torch.zeros(fem.n_dofs, dtype=dt, device=dev)
     grad_cur = grad0.clone()
     flux_cur = torch.zeros_like(grad_cur)
+    state_cur = torch.zeros(fem.n_int, fem.n_elem, fem.n_state, dtype=dt, device=dev)
     du = torch.zeros(fem.n_dofs, dtype=dt, device=dev)
-    out_u, out_f, out_q, out_g = [u_cur], [f_cur], [flux_cur], [grad_cur]
+    out_u, out_f, out_q, out_g, out_s = [u_cur], [f_cur], [flux_cur], [grad_cur], [state_cur]
 
+    mt = run.mat.tensors()
+    names, mat_tensors = tuple(mt), tuple(mt.values())
     lam = incs[0]
     step_frac = 1.0
     t0 = time.perf_counter()
@@ -324,7 +351,8 @@ def solve_finite(fem, incs: list[float], cfg: dict, nlgeom: bool, params: tuple,
                 F_ext = (lam + step) * F_unit
                 DU = step * D_unit
                 try:
-                    du = NewtonStep.apply(run, du.detach(), grad_cur, F_ext, DU, *params)
+                    du = NewtonStep.apply(run, names, du.detach(), grad_cur, flux_cur,
+                                          state_cur, F_ext, DU, *mat_tensors)
                 except RuntimeError as err:
                     step_size = cutback_factor * abs(step)
                     cutbacks += 1
@@ -334,10 +362,12 @@ def solve_finite(fem, incs: list[float], cfg: dict, nlgeom: bool, params: tuple,
                             f"after {max_cutbacks} cutbacks."
                         ) from err
                     continue
-                # converged state: forces, deformation gradient and stress
+                # converged state: forces, deformation gradient, stress, state
                 du_eval = torch.where(con, DU, du)
-                F = grad_cur + _GradOp.apply(kin, du_eval)
-                P = run.mat.stress(F)
+                H = _GradOp.apply(kin, du_eval)
+                P, state_cur, _ = run.update(H, grad_cur, flux_cur, state_cur, cfg["max_iter"],
+                                             tangent=False)
+                F = grad_cur + H
                 f_cur = _ForceOp.apply(kin, P)
                 u_cur = u_cur + du_eval
                 grad_cur = F
@@ -353,6 +383,7 @@ def solve_finite(fem, incs: list[float], cfg: dict, nlgeom: bool, params: tuple,
             out_f.append(f_cur)
             out_q.append(flux_cur)
             out_g.append(grad_cur)
+            out_s.append(state_cur)
             if verbose:
                 last = run.log[-1] if run.log else {}
                 print(f"increment {n}: load factor {lam:.6g} | newton "
@@ -363,7 +394,7 @@ def solve_finite(fem, incs: list[float], cfg: dict, nlgeom: bool, params: tuple,
     f_all = torch.stack(out_f).reshape(N, -1, nd)
     q_all = torch.stack(out_q)
     g_all = torch.stack(out_g)
-    s_all = torch.zeros(N, fem.n_int, fem.n_elem, 0, dtype=dt, device=dev)
+    s_all = torch.stack(out_s)
     if aggregate:
         q_all, g_all, s_all = q_all.mean(dim=1), g_all.mean(dim=1), s_all.mean(dim=1)
     run.close()  # a backward that needs MINRES opens its own workspace

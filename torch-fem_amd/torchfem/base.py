@@ -503,18 +503,25 @@ class FEM(ABC):
         hyperelastic material integrates F = grad_prev + grad(du) with its
         stress and tangent per Gauss point; the flux is then the Cauchy stress
         F P / J with nlgeom and P without it."""
-        if getattr(self.material, "finite_strain", False):
-            kin = Kinematics(self)
-            du = torch.as_tensor(du).reshape(-1).to(self.device, torch.float64)
-            F = grad_prev.to(self.device, torch.float64) + kin.grad(du)
-            if compute_stiffness:
-                P, A = self.material.stress_tangent(F)
-                k = kin.matrices(A)
-            else:
-                P, k = self.material.stress(F), None
-            f = kin.element_forces(P)
+        if getattr(self.material, "incremental", False):
+            from ._finite import FiniteStrainRun
+
+            run = FiniteStrainRun.__new__(FiniteStrainRun)
+            run.mat, run.kin = self.material, Kinematics(self)
+            dev, dt = self.device, torch.float64
+            du = torch.as_tensor(du).reshape(-1).to(dev, dt)
+            grad_prev = grad_prev.to(dev, dt)
+            flux_prev = (torch.zeros_like(grad_prev) if flux_prev is None
+                         else flux_prev.to(dev, dt))
+            state_prev = state_prev.to(dev, dt)
+            H = run.kin.grad(du)
+            P, state_new, A = run.update(H, grad_prev, flux_prev, state_prev, iter,
+                                         tangent=compute_stiffness)
+            k = run.kin.matrices(A) if compute_stiffness else None
+            F = grad_prev + H
+            f = run.kin.element_forces(P)
             flux = F @ P / torch.linalg.det(F)[..., None, None] if nlgeom else P
-            return k, f, F, flux, state_prev.clone()
+            return k, f, F, flux, state_new
         if nlgeom:
             raise NotImplementedError("nlgeom is supported for hyperelastic materials only")
         if de0 is not None and torch.as_tensor(de0).abs().max() > 0:
@@ -729,15 +736,15 @@ class FEM(ABC):
         ("block_jacobi" or "jacobi") and `inner_rtol` (mixed precision: the factor by
         which the FP32 recursive residual drops between two reliable updates).
         """
-        finite = getattr(self.material, "finite_strain", False)
+        finite = getattr(self.material, "incremental", False)
         if nlgeom and not finite:
             raise NotImplementedError(
-                "nlgeom is supported for hyperelastic materials (Hyperelastic3D) in this build"
+                "nlgeom is supported for hyperelastic and plastic materials in this build"
             )
         if alpha > 0.0:
             raise NotImplementedError("viscous stabilization is not part of this build")
-        if self.n_state != 0:
-            raise NotImplementedError("stateful materials are not part of this build")
+        if self.n_state != 0 and not finite:
+            raise NotImplementedError("this stateful material is not part of this build")
         if self._external_gradient.abs().max() > 0:
             raise NotImplementedError("external strains are not part of this build")
         if method not in (None, "cg", "minres"):
@@ -773,7 +780,7 @@ class FEM(ABC):
                        max_iter=max_iter, inner_rtol=inner_rtol)
             self._K_cache = None
             u_all, f_all, q_all, g_all, s_all = solve_finite(
-                self, incs, cfg, nlgeom, params, track, cutback_factor, growth_factor,
+                self, incs, cfg, nlgeom, track, cutback_factor, growth_factor,
                 max_cutbacks, aggregate_integration_points, verbose)
             if not track:
                 u_all, f_all, q_all, g_all = (t.detach() for t in (u_all, f_all, q_all, g_all))

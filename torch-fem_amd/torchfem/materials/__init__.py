@@ -14,6 +14,7 @@ the adjoint gradients flow back to them.
 from __future__ import annotations
 
 import copy
+import math
 from abc import ABC, abstractmethod
 
 import torch
@@ -28,6 +29,7 @@ __all__ = [
     "IsotropicConductivity3D",
     "IsotropicConductivity2D",
     "Hyperelastic3D",
+    "IsotropicPlasticity3D",
 ]
 
 
@@ -63,6 +65,18 @@ class Material(ABC):
 
     def rotate(self, R: Tensor) -> "Material":
         return self
+
+    def tensors(self) -> dict:
+        """The tensors an incremental update reads (the inputs through which
+        gradients reach the material)."""
+        return {}
+
+    def replace(self, **tensors) -> "Material":
+        """A shallow copy with the named tensors replaced."""
+        new = copy.copy(self)
+        for key, value in tensors.items():
+            setattr(new, key, value)
+        return new
 
     # ------------------------------------------------------------ kernels
     @abstractmethod
@@ -199,6 +213,7 @@ class Hyperelastic3D(Material):
 
     dim = 3
     finite_strain = True
+    incremental = True
 
     def __init__(self, psi, params, rho: Tensor | float = 1.0):
         self.psi = psi
@@ -217,6 +232,9 @@ class Hyperelastic3D(Material):
         new.rho = self.rho.repeat(n_elem)
         new.is_vectorized = True
         return new
+
+    def tensors(self) -> dict:
+        return {"params": self.params}
 
     def _batched(self, F: Tensor, params: Tensor | None):
         p = self.params if params is None else params
@@ -257,4 +275,90 @@ class Hyperelastic3D(Material):
         raise TypeError(
             "a hyperelastic material has no constant tangent: its models are solved "
             "by the finite-strain Newton path of FEM.solve"
+        )
+
+
+class IsotropicPlasticity3D(IsotropicElasticity3D):
+    """Small-strain J2 (von Mises) plasticity with isotropic hardening
+    (plasticity.py:24-215): yield stress sigma_f(q) of the equivalent plastic
+    strain q (one state variable), radial return with a local Newton solve for
+    the plastic multiplier, and the consistent tangent. Solved by the
+    incremental Newton path of FEM.solve, which assembles the per-point
+    tangents with the HIP kernels every iteration."""
+
+    incremental = True
+
+    def __init__(self, E, nu, sigma_f, sigma_f_prime, tolerance: float = 1e-5,
+                 max_iter: int = 10, rho: Tensor | float = 1.0):
+        super().__init__(E, nu, rho)
+        self.sigma_f = sigma_f
+        self.sigma_f_prime = sigma_f_prime
+        self.tolerance = tolerance
+        self.max_iter = max_iter
+        self.n_state = 1
+
+    def vectorize(self, n_elem: int) -> "IsotropicPlasticity3D":
+        if self.is_vectorized:
+            return self
+        new = copy.copy(self)
+        for key in ("E", "nu", "rho", "lbd", "G"):
+            v = getattr(self, key)
+            setattr(new, key, v.repeat(n_elem) if v.dim() == 0 else v)
+        new.is_vectorized = True
+        return new
+
+    def tensors(self) -> dict:
+        return {"lbd": self.lbd, "G": self.G}
+
+    def step(self, H_inc, F, stress, state, de0, cl, iter):
+        """One strain increment from (stress, state) for a batch [..., 3, 3]
+        whose trailing batch dimension runs over the elements. Returns the
+        stress, the state and the tangent [..., 3, 3, 3, 3]."""
+        dev_t = H_inc.device
+        lam = self.lbd.to(dev_t, torch.float64)
+        G = self.G.to(dev_t, torch.float64)
+        I = torch.eye(3, dtype=torch.float64, device=dev_t)
+        eps = 0.5 * (H_inc + H_inc.transpose(-1, -2))
+        if de0 is not None:
+            eps = eps - de0
+        tr = eps.diagonal(dim1=-2, dim2=-1).sum(-1)
+        trial = stress + lam[..., None, None] * tr[..., None, None] * I + 2.0 * G[..., None, None] * eps
+        s_dev = trial - trial.diagonal(dim1=-2, dim2=-1).sum(-1)[..., None, None] / 3.0 * I
+        norm = torch.sqrt((s_dev * s_dev).sum((-1, -2)))
+        q0 = state[..., 0]
+        k = math.sqrt(2.0 / 3.0)
+        yielding = norm - k * self.sigma_f(q0) > 0.0
+        # local Newton on the consistency condition
+        #   |s_dev| - 2 G dg - sqrt(2/3) sigma_f(q0 + sqrt(2/3) dg) = 0
+        dg = torch.zeros_like(norm)
+        q = q0
+        for _ in range(self.max_iter):
+            res = norm - 2.0 * G * dg - k * self.sigma_f(q)
+            step = res / (2.0 * G + (2.0 / 3.0) * self.sigma_f_prime(q))
+            dg = torch.where(yielding, dg + step, dg)
+            q = q0 + k * dg
+            if not bool((res.abs() * yielding).max() >= self.tolerance):
+                break
+        safe = torch.where(norm > 0.0, norm, torch.ones_like(norm))
+        n = s_dev / safe[..., None, None]
+        y = yielding[..., None, None]
+        new_stress = torch.where(y, trial - (2.0 * G * dg)[..., None, None] * n, trial)
+        new_state = torch.where(yielding[..., None], q[..., None], state)
+        # consistent tangent: C - a n(x)n - b (I_dev - n(x)n)
+        I4 = torch.einsum("ij,kl->ijkl", I, I)
+        I4s = 0.5 * (torch.einsum("ik,jl->ijkl", I, I) + torch.einsum("il,jk->ijkl", I, I))
+        C = (lam[..., None, None, None, None] * I4 + 2.0 * G[..., None, None, None, None] * I4s)
+        C = C.expand(*new_stress.shape[:-2], 3, 3, 3, 3)
+        a = 2.0 * G / (1.0 + self.sigma_f_prime(q) / (3.0 * G))
+        b = 4.0 * G * G * dg / safe
+        nn = torch.einsum("...ij,...kl->...ijkl", n, n)
+        soft = a[..., None, None, None, None] * nn + b[..., None, None, None, None] * (
+            I4s - I4 / 3.0 - nn)
+        tangent = torch.where(yielding[..., None, None, None, None], C - soft, C)
+        return new_stress, new_state, tangent
+
+    def kernel_params(self, n_elem, device):
+        raise TypeError(
+            "a plastic material has no constant tangent: its models are solved by the "
+            "incremental Newton path of FEM.solve"
         )

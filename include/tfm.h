@@ -207,6 +207,13 @@ int tfm_element_matrices(tfm_stream_t s, const tfm_mesh_t *m,
 int tfm_element_matrices_gp(tfm_stream_t s, const tfm_mesh_t *m, const double *tangent,
                             const double *scale, double *k, int32_t *bad_jac);
 
+/* Consistent element mass matrices mass [n_elem, npe*ndof, npe*ndof]:
+ * FEM.integrate_mass (base.py:376-396) with compute_m = rho detJ
+ * (solid.py:60-62) or rho thickness detJ (planar.py:94-96, scale = the
+ * thickness, NULL = 1); rho [n_elem]. */
+int tfm_element_mass(tfm_stream_t s, const tfm_mesh_t *m, const double *rho,
+                     const double *scale, double *mass, int32_t *bad_jac);
+
 /* Fused integration + deterministic assembly + Dirichlet epilogue: writes the
  * CSR values of K in FP64 and/or FP32 (either may be NULL) and the nodal
  * diagonal blocks. Replaces integrate_material's k plus

@@ -32,6 +32,7 @@ __all__ = [
     "OracleModel", "simp_compliance_sensitivity", "hexa_nnz_closed_form",
     "neo_hookean_psi", "hyper_step", "oracle_integrate_hyper", "oracle_solve_hyper",
     "oracle_integrate_incremental", "oracle_solve_incremental", "hyper_material", "j2_step",
+    "oracle_mass", "oracle_time_integration",
 ]
 
 
@@ -951,3 +952,70 @@ def oracle_solve_hyper(model: "OracleModel", psi, params: Tensor, constraints: T
         model, hyper_material(psi, params), 0, constraints, forces, displacements, increments,
         nlgeom, rtol, atol, max_iter)
     return u, f, flux, grad
+
+
+# --------------------------------------------------------------------------
+# Mass matrices and implicit time integration
+# (src/torchfem/base.py:376-396, 1288-1530)
+# --------------------------------------------------------------------------
+def oracle_mass(model: "OracleModel", rho: Tensor) -> Tensor:
+    """FEM.integrate_mass (base.py:376-396): m += w (rho detJ [thickness])
+    N_I N_J per DOF component. rho [E]."""
+    N, _, detJ = model.eval_shape_functions(model.etype.ipoints)
+    nd, npe = model.ndof, model.etype.nodes
+    m = torch.zeros(model.n_elem, nd * npe, nd * npe, dtype=DT)
+    I_nd = torch.eye(nd, dtype=DT)
+    for i, w in enumerate(model.etype.iweights):
+        ms = torch.einsum("N,M,E->ENM", N[i], N[i], rho * model._scale(detJ[i]))
+        m = m + float(w) * torch.einsum("Enm,ij->Enimj", ms, I_nd).reshape(m.shape)
+    return m
+
+
+def oracle_time_integration(model: "OracleModel", rho: Tensor, constraints: Tensor,
+                            prescribed: Tensor, f_ext: Tensor, t_output: Tensor, delta_t: float,
+                            rtol: float = 1e-8, atol: float = 1e-6, max_iter: int = 100):
+    """FEM.time_integration (base.py:1288-1530) for a linear material: the
+    state at t = 0 is `prescribed` everywhere; internal steps of at most
+    delta_t subdivide every interval between the output times (and 0); each
+    step solves M du + dt/2 (f_old + f_int + f_ext) = 0 by Newton with
+    (M + dt/2 K)_bc du = -R, where M and K carry the Dirichlet identity rows
+    and columns, f_int = K u (unconstrained) and f_old of the first step is
+    zero (f[0] is never filled, base.py:1410-1420). Returns u at the output
+    times [n_out, n_dofs]."""
+    con = torch.nonzero(constraints.ravel(), as_tuple=False).ravel()
+    K_el = model.k0()
+    M_el = oracle_mass(model, rho)
+    Kf = model.csr(model.assemble_matrix(K_el, torch.zeros(0, dtype=torch.int64)))
+    Mb = model.csr(model.assemble_matrix(M_el, con))
+    knots = [float(v) for v in t_output]
+    if knots[0] > 0.0:
+        knots = [0.0] + knots
+    times = [knots[0]]
+    rows = [] if float(t_output[0]) > 0.0 else [0]
+    for t0, t1 in zip(knots[:-1], knots[1:]):
+        ratio = (t1 - t0) / delta_t
+        n_sub = max(1, math.ceil(ratio - 1e-9 * max(1.0, ratio)))
+        times += list(np.linspace(t0, t1, n_sub + 1)[1:-1]) + [t1]
+        rows.append(len(times) - 1)
+    u = [prescribed.ravel().clone()]
+    f = [torch.zeros(model.n_dofs, dtype=DT)]
+    for n in range(1, len(times)):
+        dt = times[n] - times[n - 1]
+        A = model.csr(model.assemble_matrix(M_el + 0.5 * dt * K_el, con)).tocsc()
+        guess = u[-1].clone()
+        res0 = None
+        for _ in range(max_iter):
+            du = guess - u[-1]
+            f_int = torch.as_tensor(Kf @ guess.numpy(), dtype=DT)
+            R = torch.as_tensor(Mb @ du.numpy(), dtype=DT) + 0.5 * dt * (f[-1] + f_int + f_ext.ravel())
+            R[con] = 0.0
+            rn = float(torch.linalg.norm(R))
+            res0 = rn if res0 is None else res0
+            if rn < rtol * res0 or rn < atol:
+                break
+            guess = guess + torch.as_tensor(spla.spsolve(A, -R.numpy()), dtype=DT)
+        else:
+            raise RuntimeError("Newton-Raphson iteration did not converge.")
+        u.append(guess)
+        f.append(f_int)
+    return torch.stack(u)[torch.tensor(rows)]

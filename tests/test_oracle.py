@@ -7,6 +7,7 @@ closed forms, exact symbolic integration, invariants and finite differences.
 import itertools
 import math
 
+import numpy as np
 import pytest
 import sympy
 import torch
@@ -656,3 +657,53 @@ def test_j2_uniaxial_tension_closed_form():
     assert torch.allclose(u[:, 1], lat * nodes[:, 1], atol=1e-12)
     assert torch.allclose(flux[:, 0, 0], torch.full_like(flux[:, 0, 0], sig), rtol=1e-9)
     assert torch.allclose(state[:, 0], torch.full_like(state[:, 0], ep), rtol=1e-9)
+
+
+# ------------------------------------------------------- time integration
+def test_mass_matrix_partition_of_unity_and_symmetry():
+    """oracle_mass: symmetric, positive, and (N_I summing to one) its entries
+    sum to rho x volume per element and DOF component."""
+    nodes, el = O.cube_hexa(3, 4, 3)
+    inner = ((nodes > 0) & (nodes < 1)).all(dim=1)
+    nodes = nodes + 0.04 * torch.sin(4.0 * nodes[:, [1, 2, 0]]) * inner[:, None]
+    for kind, mat in (("heat", O.iso_KAPPA(torch.ones(len(el), dtype=DT), 3)),
+                      ("solid", O.iso3d_C(torch.ones(len(el), dtype=DT), 0.3))):
+        m = O.OracleModel(nodes, el, O.Hexa1, kind, mat)
+        rho = 1.0 + torch.rand(len(el), generator=torch.Generator().manual_seed(0), dtype=DT)
+        M = O.oracle_mass(m, rho)
+        _, _, detJ = m.eval_shape_functions(O.Hexa1.ipoints)
+        V = (detJ * O.Hexa1.iweights[:, None]).sum(0)
+        assert torch.allclose(M, M.transpose(1, 2), atol=1e-15)
+        assert torch.allclose(M.sum((1, 2)), m.ndof * rho * V, rtol=1e-13)
+        assert bool((torch.linalg.eigvalsh(M) > 0).all())
+
+
+def test_time_integration_decays_an_eigenmode_exactly():
+    """oracle_time_integration on a generalized eigenvector K phi = lam M phi
+    (zero boundary temperature): the first step, whose old internal vector
+    is zero as in the reference, gives phi / (1 + dt lam / 2); every later
+    (Crank-Nicolson) step multiplies by (1 - dt lam / 2) / (1 + dt lam / 2)."""
+    import scipy.linalg as sla
+
+    nodes, el = O.cube_hexa(5, 5, 5)
+    kappa, rho = 2.0, 3.0
+    m = O.OracleModel(nodes, el, O.Hexa1, "heat", O.iso_KAPPA(torch.full((len(el),), kappa,
+                                                                        dtype=DT), 3))
+    bnd = _boundary(nodes)
+    free = torch.nonzero(~bnd).ravel()
+    K = m.csr(m.assemble_matrix(m.k0(), torch.zeros(0, dtype=torch.int64))).toarray()
+    M = m.csr(m.assemble_matrix(O.oracle_mass(m, torch.full((len(el),), rho, dtype=DT)),
+                                torch.zeros(0, dtype=torch.int64))).toarray()
+    f = free.numpy()
+    lam, vec = sla.eigh(K[np.ix_(f, f)], M[np.ix_(f, f)])
+    phi = torch.zeros(len(nodes), dtype=DT)
+    phi[free] = torch.as_tensor(vec[:, 1])
+    dt = 0.01
+    T = O.oracle_time_integration(m, torch.full((len(el),), rho, dtype=DT), bnd[:, None],
+                                  phi[:, None], torch.zeros(len(nodes), 1, dtype=DT),
+                                  torch.tensor([0.0, 0.03, 0.06]), dt, rtol=1e-13, atol=1e-14)
+    a = 0.5 * dt * lam[1]
+    g = (1 - a) / (1 + a)
+    for row, n in ((0, 0), (1, 3), (2, 6)):
+        amp = 1.0 if n == 0 else g ** (n - 1) / (1 + a)
+        assert torch.allclose(T[row], amp * phi, atol=1e-12 * float(phi.abs().max()))

@@ -493,6 +493,52 @@ __global__ void k_body_load(tfm_mesh_t m, const double *scale, const double *loa
     for (int c = 0; c < K; ++c) fe[(e * E::NPE + i) * K + c] = wn[i] * sc * lv[c];
 }
 
+// Consistent element mass matrices (base.py:376-396): one 64-lane block per
+// element stages N_I and w detJ rho_e scale_e at every Gauss point in LDS,
+// then m[(I,a),(J,b)] = delta_ab sum_g w detJ rho scale N_I N_J.
+template <class E, int ND>
+__global__ void __launch_bounds__(64) k_element_mass(tfm_mesh_t m, const double *rho,
+                                                     const double *scale, double *mout,
+                                                     int32_t *bad) {
+  constexpr int NPE = E::NPE, NGP = E::NGP, D = E::DIM, W = NPE * ND;
+  __shared__ double X[NPE][D];
+  __shared__ double Nv[NGP][NPE];
+  __shared__ double wd[NGP];
+  const int64_t e = xcd_block();
+  const int tid = threadIdx.x;
+  for (int t = tid; t < NPE * D; t += 64) {
+    const int j = t / D, d = t - j * D;
+    X[j][d] = m.nodes[(int64_t)m.elements[e * NPE + j] * D + d];
+  }
+  __syncthreads();
+  for (int g = tid; g < NGP; g += 64) {
+    double xi[3] = {0.0, 0.0, 0.0}, w = 0.0, N[NPE], dN[D][NPE];
+    E::gauss(g, xi, w);
+    E::dshape(xi, dN);
+    shape_values<E>(xi, N);
+    double J[D][D], Ji[D][D];
+    for (int i = 0; i < D; ++i)
+      for (int k = 0; k < D; ++k) {
+        double acc = 0.0;
+        for (int I = 0; I < NPE; ++I) acc += dN[i][I] * X[I][k];
+        J[i][k] = acc;
+      }
+    const double det = inv_jac<D>(J, Ji);
+    if (!(det > 0.0)) *bad = 1;
+    for (int I = 0; I < NPE; ++I) Nv[g][I] = N[I];
+    wd[g] = w * det * rho[e] * (scale ? scale[e] : 1.0);
+  }
+  __syncthreads();
+  for (int t = tid; t < W * W; t += 64) {
+    const int r = t / W, c = t - r * W;
+    const int I = r / ND, a = r - I * ND, J = c / ND, b = c - J * ND;
+    double v = 0.0;
+    if (a == b)
+      for (int g = 0; g < NGP; ++g) v += wd[g] * Nv[g][I] * Nv[g][J];
+    mout[e * W * W + t] = v;
+  }
+}
+
 // B[q][e][k][I] = (J^{-1} dN/dxi)[k][I] and detJ[q][e] at local points xi[q]
 // (the quantities of base.py:293-314 at arbitrary points).
 template <class E>
@@ -674,6 +720,25 @@ extern "C" int tfm_body_load(tfm_stream_t stream, const tfm_mesh_t *m, const dou
       mk.ndof = K;
       return launch_gather(s, mk, fe, F);
     });
+  });
+}
+
+extern "C" int tfm_element_mass(tfm_stream_t stream, const tfm_mesh_t *m, const double *rho,
+                                const double *scale, double *mass, int32_t *bad_jac) {
+  TFM_REQUIRE(m && rho && mass && bad_jac, "null argument");
+  TFM_REQUIRE(m->ndof >= 1 && m->ndof <= 3, "ndof must be 1, 2 or 3");
+  if (m->n_elem == 0) return TFM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  return with_etype(m->etype, [&](auto et) -> int {
+    using E = decltype(et);
+    const dim3 g((unsigned)m->n_elem);
+    switch (m->ndof) {
+      case 1: hipLaunchKernelGGL((k_element_mass<E, 1>), g, dim3(64), 0, s, *m, rho, scale, mass, bad_jac); break;
+      case 2: hipLaunchKernelGGL((k_element_mass<E, 2>), g, dim3(64), 0, s, *m, rho, scale, mass, bad_jac); break;
+      default: hipLaunchKernelGGL((k_element_mass<E, 3>), g, dim3(64), 0, s, *m, rho, scale, mass, bad_jac); break;
+    }
+    TFM_CHECK_LAUNCH();
+    return TFM_OK;
   });
 }
 

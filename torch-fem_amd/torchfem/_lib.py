@@ -105,6 +105,7 @@ _SIGNATURES: dict[str, tuple[list[Any], Any]] = {
     "tfm_pattern_expand": ([P, C.POINTER(Mesh), P, P, P, P, P], C.c_int),
     "tfm_element_matrices": ([P, C.POINTER(Mesh), C.POINTER(Material), P, P], C.c_int),
     "tfm_element_matrices_gp": ([P, C.POINTER(Mesh), P, P, P, P], C.c_int),
+    "tfm_element_mass": ([P, C.POINTER(Mesh), P, P, P, P], C.c_int),
     "tfm_assemble": (
         [P, C.POINTER(Mesh), C.POINTER(Material), P, P, P, P, P, I32],
         C.c_int,

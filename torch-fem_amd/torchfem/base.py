@@ -24,6 +24,7 @@ which is exact for a linear model.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import time
 from abc import ABC, abstractmethod
 from collections.abc import Iterable
@@ -865,6 +866,159 @@ class FEM(ABC):
         if return_intermediate:
             return u_all, f_all, q_all, g_all, s_all
         return u_all[-1], f_all[-1], q_all[-1], g_all[-1], s_all[-1]
+
+    def integrate_mass(self) -> Tensor:
+        """Consistent element mass matrices [n_elem, npe*ndof, npe*ndof]
+        (base.py:376-396): sum over Gauss points of w detJ rho (thickness)
+        N_I N_J per DOF component, one HIP block per element."""
+        if self.material is None:
+            raise ValueError("this model has no material")
+        rho = torch.as_tensor(self.material.rho).to(self.device, torch.float64)
+        rho = torch.broadcast_to(rho.reshape(-1) if rho.dim() else rho, (self.n_elem,)).contiguous()
+        scale = self._scale_tensor()
+        pat = self._pattern
+        W = pat.npe * pat.ndof
+        m = torch.empty(self.n_elem, W, W, dtype=torch.float64, device=self.device)
+        bad = torch.zeros(1, dtype=torch.int32, device=self.device)
+        L.check(L.lib().tfm_element_mass(pat.stream(), C.byref(pat.mesh), L.ptr(rho),
+                                         L.ptr(scale.detach() if scale is not None else None),
+                                         L.ptr(m), L.ptr(bad)))
+        self._check_jacobian(bad)
+        return m
+
+    def time_integration(
+        self,
+        t_output: Tensor | None = None,
+        delta_t: float = 1.0e-1,
+        max_iter: int = 100,
+        verbose: bool = False,
+        rtol: float = 1e-8,
+        atol: float = 1e-6,
+        stol: float = 1e-10,
+        device: str | None = None,
+        method: str | None = None,
+        aggregate_integration_points: bool = True,
+        use_cached_solve: bool = False,
+        differentiable_parameters: Tensor | Iterable[Tensor] | None = None,
+        precision: str | None = None,
+        preconditioner: str = "block_jacobi",
+        inner_rtol: float = 0.1,
+    ):
+        """Implicit time integration from an equilibrium state
+        (base.py:1288-1530): the state at t = 0 is the solution with every
+        DOF at its prescribed value; each internal step of at most delta_t
+        (the output times are hit exactly) solves
+        M du + dt/2 (f_int_old + f_int(du) + f_ext) = 0 by Newton with
+        (M + dt/2 K) du = -R on the node-block PCG, M being the consistent
+        mass (integrate_mass, a HIP kernel) assembled like K. Returns
+        temperature (or displacement), internal vector, flux, gradient and
+        state at the output times. Gradients through the time steps are not
+        part of this build."""
+        t_output = torch.tensor([0.0, 1.0]) if t_output is None else torch.as_tensor(t_output)
+        t_output = t_output.to(torch.float64).reshape(-1)
+        if t_output.numel() == 0:
+            raise ValueError("t_output must contain at least one time.")
+        if t_output.min() < 0.0:
+            raise ValueError("t_output must not contain negative times.")
+        if (t_output[1:] <= t_output[:-1]).any():
+            raise ValueError("t_output must be strictly increasing.")
+        if differentiable_parameters is not None:
+            params = ((differentiable_parameters,) if isinstance(differentiable_parameters, Tensor)
+                      else tuple(differentiable_parameters))
+            if any(p.requires_grad for p in params):
+                raise NotImplementedError(
+                    "gradients through time integration are not part of this build")
+        if getattr(self.material, "incremental", False) or self.n_state != 0:
+            raise NotImplementedError("time integration supports linear materials in this build")
+        precision = precision or default_precision()
+        cfg = dict(precision=precision, pc=preconditioner, stol=stol, inner_rtol=inner_rtol)
+        dev, dt64 = self.device, torch.float64
+        nd = self.n_dof_per_node
+
+        # equilibrium at t = 0 with every DOF prescribed
+        bc = self._constraints.clone()
+        self._constraints = torch.ones_like(bc)
+        try:
+            T0 = self.solve(precision=precision, stol=stol)[0].reshape(-1)
+        finally:
+            self._constraints = bc
+        shape = (self.n_int, self.n_elem, *self.n_flux)
+        zero_q = torch.zeros(shape, dtype=dt64, device=dev)
+        g_init = self.initial_grad.to(dev, dt64).expand(shape).clone()
+        s0 = torch.zeros(self.n_int, self.n_elem, self.n_state, dtype=dt64, device=dev)
+        _, _, g0, q0, _ = self.integrate_material(None, g_init, zero_q, s0, T0, None, 0, False,
+                                                  compute_stiffness=False)
+
+        # internal times: every interval between output times in equal steps
+        # of at most delta_t, starting at t = 0
+        knots = [float(v) for v in t_output]
+        if knots[0] > 0.0:
+            knots = [0.0] + knots
+        times, rows = [knots[0]], ([] if float(t_output[0]) > 0.0 else [0])
+        for t0, t1 in zip(knots[:-1], knots[1:]):
+            ratio = (t1 - t0) / delta_t
+            n_sub = max(1, math.ceil(ratio - 1e-9 * max(1.0, ratio)))
+            times += [t0 + (t1 - t0) * (k + 1) / n_sub for k in range(n_sub - 1)] + [t1]
+            rows.append(len(times) - 1)
+
+        con = self._con_mask().bool()
+        f_ext = self._neumann.reshape(-1).to(dt64)
+        m_el = self.integrate_mass()
+        k_el = self.k0()
+        from ._finite import TangentSystem
+
+        M = TangentSystem(self, m_el, "fp64")  # M with the Dirichlet rows and columns
+        systems: dict = {}
+        u = [T0]
+        f = [torch.zeros_like(T0)]
+        q = [q0]
+        g = [g0]
+        st = [s0]
+        t_start = time.perf_counter()
+        info = []
+        for n in range(1, len(times)):
+            dt = times[n] - times[n - 1]
+            key = round(dt, 15)
+            if key not in systems:
+                systems[key] = TangentSystem(self, m_el + 0.5 * dt * k_el, precision)
+            A = systems[key]
+            guess = u[-1].clone()
+            res0 = None
+            for it in range(max_iter):
+                du = guess - u[-1]
+                _, fe, g_new, q_new, s_new = self.integrate_material(
+                    u[-1], g[-1], q[-1], st[-1], du, None, it, False, compute_stiffness=False)
+                f_int = self.assemble_rhs(fe)
+                Mdu = torch.empty_like(du)
+                L.check(L.lib().tfm_spmv(self._pattern.stream(), C.byref(self._pattern.mesh), 0,
+                                         L.ptr(M.val64), L.ptr(du.contiguous()), L.ptr(Mdu), 1))
+                res = Mdu + 0.5 * dt * (f[-1] + f_int + f_ext)
+                res = torch.where(con, torch.zeros_like(res), res)
+                rn = float(torch.linalg.norm(res))
+                res0 = rn if res0 is None else res0
+                if rn < rtol * res0 or rn < atol:
+                    break
+                guess = guess + self._pcg(A, -res[None], cfg)[0]
+            else:
+                raise RuntimeError("Newton-Raphson iteration did not converge.")
+            info.append(dict(time=times[n], newton_iterations=it, res=rn))
+            if verbose:
+                print(f"time step {n}: t = {times[n]:.6g} | newton {it} | res {rn:.3e}", flush=True)
+            u.append(guess)
+            f.append(f_int)
+            q.append(q_new)
+            g.append(g_new)
+            st.append(s_new)
+        self.last_solve_info = dict(time_integration=info, time=time.perf_counter() - t_start)
+        idx = torch.tensor(rows)
+        out_u = torch.stack(u)[idx].reshape(len(rows), -1, nd)
+        out_f = torch.stack(f)[idx].reshape(len(rows), -1, nd)
+        out_q = torch.stack(q)[idx]
+        out_g = torch.stack(g)[idx]
+        out_s = torch.stack(st)[idx]
+        if aggregate_integration_points:
+            out_q, out_g, out_s = out_q.mean(dim=1), out_g.mean(dim=1), out_s.mean(dim=1)
+        return out_u, out_f, out_q.squeeze((-2, -1)), out_g.squeeze((-2, -1)), out_s
 
     def solve_load_cases(
         self,

@@ -14,6 +14,7 @@ index arrays (glob_idx, k_map, diag_map) are exact expansions of it
 from __future__ import annotations
 
 import ctypes as C
+from dataclasses import dataclass
 
 import numpy as np
 import torch
@@ -168,3 +169,31 @@ class DevicePattern:
             )
         )
         return out
+
+
+def locality_order(nodes: Tensor, n_elem: int) -> Tensor:
+    """A node order with neighbours close together: nodes sorted by the bins
+    of a grid whose spacing is the mean element size, the axis with the most
+    bins varying slowest (the thinnest slabs), ties kept in the mesh's order.
+    On a structured mesh this is the lexicographic order with mid-side nodes
+    beside their corners. Returns perm (new index -> mesh index), on the
+    nodes' device."""
+    x = nodes - nodes.min(dim=0).values
+    ext = x.max(dim=0).values
+    d = nodes.shape[1]
+    vol = float(torch.prod(torch.clamp(ext, min=1e-300)))
+    h = (vol / max(n_elem, 1)) ** (1.0 / d)
+    bins = torch.floor(x / h).to(torch.int64)
+    nb = bins.max(dim=0).values + 1
+    axes = torch.argsort(nb, descending=True).tolist()
+    key = torch.zeros(nodes.shape[0], dtype=torch.int64, device=nodes.device)
+    for a in axes:
+        key = key * int(nb[a]) + bins[:, a]
+    return torch.argsort(key, stable=True)
+
+
+@dataclass
+class SolverView:
+    """A pattern in the solver's node order and the map solver DOF -> model DOF."""
+    pattern: DevicePattern
+    perm_dof: Tensor

@@ -34,7 +34,7 @@ import torch
 from torch import Tensor
 
 from . import _lib as L
-from ._pattern import DevicePattern
+from ._pattern import DevicePattern, SolverView, locality_order
 from .elements import Element
 from .materials import Material
 from ._finite import Kinematics, solve_finite
@@ -54,8 +54,14 @@ class _System:
     diagonal blocks, the Dirichlet mask and the material it was built from."""
 
     def __init__(self, fem: "FEM", kind: int, p0: Tensor, p1: Tensor | None,
-                 scale: Tensor | None, precision: str, nrhs: int = 1):
-        pat = fem._pattern
+                 scale: Tensor | None, precision: str, nrhs: int = 1, renumber: bool = True):
+        # the solver's node order: the mesh's own, or a locality order when the
+        # mesh numbering scatters neighbours (FEM._solver_view); `perm` maps a
+        # solver DOF to the model DOF, and FEM._pcg translates vectors
+        view = fem._solver_view() if renumber else None
+        pat = view.pattern if view is not None else fem._pattern
+        self.perm = view.perm_dof if view is not None else None
+        self._pat = pat
         dev = pat.device
         self.fem = fem
         self.kind = kind
@@ -65,7 +71,9 @@ class _System:
         self.mat = L.Material(kind=kind, p0=self.p0.data_ptr(),
                               p1=self.p1.data_ptr() if self.p1 is not None else None,
                               scale=self.scale.data_ptr() if self.scale is not None else None)
-        self.con = fem._con_mask()
+        self.con = fem._con_mask()  # model order
+        con_s = self.con[self.perm].contiguous() if self.perm is not None else self.con
+        self._con_s = con_s
         self.precision = precision
         # mixed precision keeps the FP64 values too: its refinement residuals
         # b - K x are then one FP64 SpMV instead of a matrix-free pass
@@ -78,11 +86,11 @@ class _System:
         # form the solver's FP32 products take for that many), plain for more
         form = L.FP32_DIFF if nrhs <= 2 else L.FP32_PLAIN
         L.check(L.lib().tfm_assemble(pat.stream(), C.byref(pat.mesh), C.byref(self.mat),
-                                     L.ptr(self.con), L.ptr(self.val64), L.ptr(self.val32),
+                                     L.ptr(con_s), L.ptr(self.val64), L.ptr(self.val32),
                                      L.ptr(self.diag), L.ptr(bad), form))
         fem._check_jacobian(bad)
         self.sys = L.System(mesh=C.pointer(pat.mesh), mat=C.pointer(self.mat),
-                            con=self.con.data_ptr(),
+                            con=con_s.data_ptr(),
                             val64=L.ptr(self.val64), val32=L.ptr(self.val32),
                             diag_blocks=self.diag.data_ptr(), val32_form=form)
 
@@ -275,6 +283,8 @@ class FEM(ABC):
             self.material = material.vectorize(self.n_elem)
         self.cached_solve = CachedSolve()
         self._pcg_handle: PCGSolver | None = None
+        self._sv_checked = False
+        self._sv = None
         self._last_system: _System | None = None
         self._K_cache = None
         self.last_solve_info: dict = {}
@@ -438,13 +448,48 @@ class FEM(ABC):
         enters the gradient, at 0.7 stol."""
         b = b.to(torch.float64).contiguous()
         x = torch.zeros_like(b) if warm is None else warm.reshape(b.shape).to(torch.float64).clone()
+        perm = getattr(system, "perm", None)
+        if perm is not None:  # to the solver's node order
+            b, x = b[:, perm].contiguous(), x[:, perm].contiguous()
         solver = self._solver()
         info = solver.solve_system(system.sys, b, x, precision=cfg["precision"],
                                    preconditioner=cfg["pc"], rtol=cfg["stol"],
                                    inner_rtol=cfg["inner_rtol"],
                                    final_rtol=(0.7 if adjoint else 0.25) * cfg["stol"])
         self._last_pcg = info
+        if perm is not None:
+            out = torch.empty_like(x)
+            out[:, perm] = x
+            return out
         return x
+
+    def _solver_view(self):
+        """The node order the PCG runs in. A mesh whose numbering puts
+        neighbours far apart (16-bit neighbour offsets do not fit, e.g. the
+        mid-side nodes that linear_to_quadratic appends after the corners)
+        scatters the SpMV's gathers over HBM; the solver then works on a copy
+        of the pattern in a locality order (`locality_order`: nodes sorted by
+        element-size bins of their coordinates) when that order brings every
+        offset within 16 bits. Elements keep their order, so material arrays
+        and element outputs are unchanged; only nodal vectors are permuted,
+        in FEM._pcg. None when the mesh's own order is kept."""
+        if self._sv_checked:
+            return self._sv
+        self._sv_checked = True
+        pat = self._pattern
+        if pat.adj16 is not None or self.n_nod < 2 or self.n_elem == 0:
+            return None
+        perm = locality_order(self.nodes, self.n_elem)
+        inv = torch.empty_like(perm)
+        inv[perm] = torch.arange(self.n_nod, device=self.device)
+        spat = DevicePattern(self.nodes[perm], inv[self.elements], self.n_dof_per_node,
+                             self.etype.__name__, self.device)
+        if spat.adj16 is None:
+            return None
+        nd = self.n_dof_per_node
+        perm_dof = (perm[:, None] * nd + torch.arange(nd, device=self.device)).reshape(-1)
+        self._sv = SolverView(spat, perm_dof.contiguous())
+        return self._sv
 
     def _newton(self, system: _System, cfg: dict, F_ext: Tensor, u0: Tensor, con: Tensor) -> Tensor:
         """Newton loop of NewtonRaphsonAdjoint.forward (sparse.py:541-639) on the
@@ -582,10 +627,10 @@ class FEM(ABC):
         s = self._last_system
         if s is None:
             return torch.empty(0, device=self.device)
-        if s.val64 is not None:
+        if s.val64 is not None and s.perm is None:
             val = s.val64
-        else:  # mixed solves keep FP32 values only: assemble FP64 on demand
-            s64 = _System(self, s.kind, s.p0, s.p1, s.scale, "fp64")
+        else:  # values in the solver's node order: assemble in the model's
+            s64 = _System(self, s.kind, s.p0, s.p1, s.scale, "fp64", renumber=False)
             val = s64.val64
         with torch.sparse.check_sparse_tensor_invariants(False):
             self._K_cache = torch.sparse_coo_tensor(self.glob_idx, val,

@@ -32,7 +32,7 @@ __all__ = [
     "OracleModel", "simp_compliance_sensitivity", "hexa_nnz_closed_form",
     "neo_hookean_psi", "hyper_step", "oracle_integrate_hyper", "oracle_solve_hyper",
     "oracle_integrate_incremental", "oracle_solve_incremental", "hyper_material", "j2_step",
-    "oracle_mass", "oracle_time_integration",
+    "oracle_mass", "oracle_time_integration", "hyper_plane_strain", "hyper_plane_stress",
 ]
 
 
@@ -802,10 +802,10 @@ def oracle_integrate_incremental(model: "OracleModel", step, grad_prev: Tensor, 
     step(H, grad_prev, flux_prev, state_prev), f += w detJ B^T stress,
     k += w detJ B^T A B, F = grad_prev + H. Returns k, f, F, stress, state."""
     _, B, detJ = model.eval_shape_functions(model.etype.ipoints)
-    E, npe = model.n_elem, model.etype.nodes
-    due = du.view(-1, 3)[model.elements].reshape(E, -1, 3).transpose(-1, -2)
-    f = torch.zeros(E, 3 * npe, dtype=DT)
-    k = torch.zeros(E, 3 * npe, 3 * npe, dtype=DT) if need_k else None
+    E, npe, d = model.n_elem, model.etype.nodes, model.ndof
+    due = du.view(-1, d)[model.elements].reshape(E, -1, d).transpose(-1, -2)
+    f = torch.zeros(E, d * npe, dtype=DT)
+    k = torch.zeros(E, d * npe, d * npe, dtype=DT) if need_k else None
     Fs, Ps, Ss = [], [], []
     for i, w in enumerate(model.etype.iweights):
         H = due @ B[i].transpose(-1, -2)
@@ -813,11 +813,12 @@ def oracle_integrate_incremental(model: "OracleModel", step, grad_prev: Tensor, 
         Fs.append(grad_prev[i] + H)
         Ps.append(P)
         Ss.append(state)
-        fc = torch.einsum("..., ...iI,...Ai->...IA", detJ[i], B[i], P)
-        f = f + float(w) * fc.reshape(-1, 3 * npe)
+        sdet = model._scale(detJ[i])  # detJ, times the thickness for planar models
+        fc = torch.einsum("..., ...iI,...Ai->...IA", sdet, B[i], P)
+        f = f + float(w) * fc.reshape(-1, d * npe)
         if need_k:
             BCB = torch.einsum("...Jp,...iJkL,...Lq->...piqk", B[i], A, B[i])
-            k = k + torch.einsum("j,jkl->jkl", detJ[i], BCB.reshape(-1, 3 * npe, 3 * npe)) * float(w)
+            k = k + torch.einsum("j,jkl->jkl", sdet, BCB.reshape(-1, d * npe, d * npe)) * float(w)
     return k, f, torch.stack(Fs), torch.stack(Ps), torch.stack(Ss)
 
 
@@ -826,6 +827,42 @@ def hyper_material(psi, params: Tensor):
     def step(H, F_prev, stress, state):
         P, A = hyper_step(psi, F_prev + H, params)
         return P, state, A
+    return step
+
+
+def _embed2(F2: Tensor, f33) -> Tensor:
+    F3 = torch.zeros(*F2.shape[:-2], 3, 3, dtype=DT)
+    F3[..., :2, :2] = F2
+    F3[..., 2, 2] = f33
+    return F3
+
+
+def hyper_plane_strain(psi, params: Tensor):
+    """HyperelasticPlaneStrain (hyperelasticity.py:272-330): the 3-D energy at
+    diag(F2, 1), in-plane blocks of P and of the tangent."""
+    def step(H, F_prev, stress, state):
+        P, A = hyper_step(psi, _embed2(F_prev + H, 1.0), params)
+        return P[..., :2, :2], state, A[..., :2, :2, :2, :2]
+    return step
+
+
+def hyper_plane_stress(psi, params: Tensor, tol: float = 1e-12, max_iter: int = 50):
+    """HyperelasticPlaneStress.step (hyperelasticity.py:191-269): Newton on
+    the out-of-plane stretch for P33 = 0 from the stored lam3 - 1, then the
+    condensed tangent A_ijkl - A_ij33 A_33kl / A_3333."""
+    def step(H, F_prev, stress, state):
+        F2 = F_prev + H
+        lam3 = 1.0 + state[..., 0]
+        for _ in range(max_iter):
+            P, A = hyper_step(psi, _embed2(F2, lam3), params)
+            r = P[..., 2, 2]
+            lam3 = lam3 - r / A[..., 2, 2, 2, 2]
+            if float(r.abs().max()) < tol:
+                break
+        A2 = A[..., :2, :2, :2, :2] - torch.einsum(
+            "...ij,...kl->...ijkl", A[..., :2, :2, 2, 2], A[..., 2, 2, :2, :2]) / A[
+            ..., 2, 2, 2, 2][..., None, None, None, None]
+        return P[..., :2, :2], (lam3 - 1.0)[..., None], A2
     return step
 
 
@@ -904,8 +941,9 @@ def oracle_solve_incremental(model: "OracleModel", step, n_state: int, constrain
     con = torch.nonzero(constraints.ravel(), as_tuple=False).ravel()
     n = model.n_dofs
     n_gp = len(model.etype.iweights)
-    grad = torch.eye(3, dtype=DT).expand(n_gp, model.n_elem, 3, 3).clone()
-    flux = torch.zeros(n_gp, model.n_elem, 3, 3, dtype=DT)
+    d = model.ndof
+    grad = torch.eye(d, dtype=DT).expand(n_gp, model.n_elem, d, d).clone()
+    flux = torch.zeros(n_gp, model.n_elem, d, d, dtype=DT)
     state = torch.zeros(n_gp, model.n_elem, n_state, dtype=DT)
     u = torch.zeros(n, dtype=DT)
     du = torch.zeros(n, dtype=DT)
@@ -940,7 +978,7 @@ def oracle_solve_incremental(model: "OracleModel", step, n_state: int, constrain
         f_int = model.assemble_rhs(fe)
         u = u + du
         lam_prev = lam_n
-    return u.reshape(-1, 3), f_int.reshape(-1, 3), flux.mean(0), grad.mean(0), state.mean(0)
+    return u.reshape(-1, d), f_int.reshape(-1, d), flux.mean(0), grad.mean(0), state.mean(0)
 
 
 def oracle_solve_hyper(model: "OracleModel", psi, params: Tensor, constraints: Tensor,

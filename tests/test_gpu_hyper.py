@@ -262,3 +262,90 @@ def test_nlgeom_for_linear_material_is_rejected():
     fem = Solid(nodes.cuda(), el.cuda(), IsotropicElasticity3D(1000.0, 0.3))
     with pytest.raises(NotImplementedError):
         fem.solve(nlgeom=True)
+
+
+# ------------------------------------------------ planar hyperelasticity
+from test_oracle import plane_strain_uniaxial_closed_form  # noqa: E402
+from torchfem import Planar  # noqa: E402
+from torchfem.materials import HyperelasticPlaneStrain, HyperelasticPlaneStress  # noqa: E402
+
+
+def _quad_mesh():
+    nodes, el = O.rect_quad(5, 4, 2.0, 1.0)
+    inner = (nodes[:, 0] > 0) & (nodes[:, 0] < 2) & (nodes[:, 1] > 0) & (nodes[:, 1] < 1)
+    return nodes + 0.05 * torch.sin(5.0 * nodes[:, [1, 0]]) * inner[:, None], el
+
+
+@pytest.mark.parametrize("kind", ["strain", "stress"])
+def test_planar_integrate_material_matches_oracle(kind):
+    """k, f, F, stress and state of one planar finite-strain evaluation with
+    per-element parameters and thickness, from a previous deformation
+    gradient (plane stress: and a previous out-of-plane stretch)."""
+    nodes, el = _quad_mesh()
+    params = _params(len(el))
+    t = 0.5 + torch.rand(len(el), generator=torch.Generator().manual_seed(2), dtype=DT)
+    cls = HyperelasticPlaneStress if kind == "stress" else HyperelasticPlaneStrain
+    kw = dict(tolerance=1e-13, max_iter=50) if kind == "stress" else {}
+    fem = Planar(nodes.cuda(), el.cuda(), cls(psi, params, **kw), thickness=t.cuda())
+    m = O.OracleModel(nodes, el, O.Quad1, "planar",
+                      O.plane_stress_C(torch.full((len(el),), 1.0), torch.tensor(0.3)),
+                      thickness=t)
+    g = torch.Generator().manual_seed(3)
+    ngp = fem.n_int
+    du = 0.05 * torch.randn(len(nodes) * 2, generator=g, dtype=DT)
+    grad_prev = torch.eye(2, dtype=DT) + 0.05 * torch.randn(ngp, len(el), 2, 2, generator=g,
+                                                              dtype=DT)
+    state = (0.05 * torch.randn(ngp, len(el), 1, generator=g, dtype=DT) if kind == "stress"
+             else torch.zeros(ngp, len(el), 0, dtype=DT))
+    k, f, F, flux, st = fem.integrate_material(
+        None, grad_prev.cuda(), None, state.cuda(), du.cuda(), None, 0, False)
+    step = (O.hyper_plane_stress(O.neo_hookean_psi, params, tol=1e-13) if kind == "stress"
+            else O.hyper_plane_strain(O.neo_hookean_psi, params))
+    k_o, f_o, F_o, P_o, st_o = O.oracle_integrate_incremental(
+        m, step, grad_prev, torch.zeros(ngp, len(el), 2, 2, dtype=DT), state, du)
+    assert rel(F, F_o) < 1e-14
+    assert rel(flux, P_o) < 1e-11
+    assert rel(f, f_o) < 1e-11
+    assert rel(k, k_o) < 1e-10
+    if kind == "stress":
+        assert rel(st, st_o) < 1e-11
+
+
+@pytest.mark.parametrize("kind", ["strain", "stress"])
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_planar_uniaxial_stretch_at_size(kind, precision):
+    """A Quad1 plate of thickness 0.3 stretched to 1.8x: plane stress (40 x 40
+    cells) gives the 3-D uniaxial closed form and its gradient w.r.t.
+    (mu, lbd) (times the thickness), plane strain (10 x 10 cells) the closed
+    form with F_zz = 1. The first Newton iterate of an increment puts the
+    whole increment into the last column of cells; in plane strain, with no
+    out-of-plane contraction, J there grows with the refinement until the
+    Neo-Hookean tangent turns indefinite and Newton from that iterate (the
+    reference's scheme, which the oracle's restatement shares) diverges
+    beyond about 20 x 20 cells."""
+    mu, lbd, t = 350.0, 800.0, 0.3
+    n = 41 if kind == "stress" else 11
+    nodes, el = O.rect_quad(n, n, 1.0, 1.0)
+    params = torch.tensor([mu, lbd], dtype=DT, requires_grad=True)
+    cls = HyperelasticPlaneStress if kind == "stress" else HyperelasticPlaneStrain
+    kw = dict(tolerance=1e-12, max_iter=50) if kind == "stress" else {}
+    plate = Planar(nodes.cuda(), el.cuda(), cls(psi, params, **kw),
+                   thickness=torch.full((len(el),), t, dtype=DT).cuda())
+    X = plate.nodes
+    right = X[:, 0] == 1.0
+    plate.constraints[X[:, 0] == 0.0, 0] = True
+    plate.constraints[X[:, 1] == 0.0, 1] = True
+    plate.constraints[right, 0] = True
+    plate.displacements[right, 0] = 0.8
+    u, f, _, _, _ = plate.solve(increments=torch.tensor([0.0, 0.25, 0.5, 0.75, 1.0]),
+                                nlgeom=True, differentiable_parameters=params,
+                                precision=precision)
+    reaction = f[right, 0].sum()
+    if kind == "stress":
+        R, dR = uniaxial_stretch_closed_form(mu, lbd, 1.8)
+        (grad,) = torch.autograd.grad(reaction, params)
+        assert torch.allclose(grad, t * torch.tensor(dR, dtype=DT), rtol=1e-6)
+    else:
+        R, lat = plane_strain_uniaxial_closed_form(mu, lbd, 1.8)
+        assert (u[:, 1] - (lat - 1.0) * X[:, 1]).abs().max() < 1e-9
+    assert abs(float(reaction) - R * t) < 1e-7 * R * t

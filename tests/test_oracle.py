@@ -707,3 +707,61 @@ def test_time_integration_decays_an_eigenmode_exactly():
     for row, n in ((0, 0), (1, 3), (2, 6)):
         amp = 1.0 if n == 0 else g ** (n - 1) / (1 + a)
         assert torch.allclose(T[row], amp * phi, atol=1e-12 * float(phi.abs().max()))
+
+
+# ------------------------------------------------ planar hyperelasticity
+def plane_strain_uniaxial_closed_form(mu, lbd, lam):
+    """Neo-Hookean plate in plane strain stretched by lam along x, free
+    along y: F = diag(lam, s, 1) with P_yy = mu (s - 1/s) + lbd ln(lam s) / s
+    = 0 (scalar Newton); the nominal stress P_xx = mu (lam - 1/lam)
+    + lbd ln(lam s) / lam."""
+    s = 1.0
+    for _ in range(100):
+        r = mu * (s - 1.0 / s) + lbd * math.log(lam * s) / s
+        dr = mu * (1.0 + 1.0 / s**2) + lbd * (1.0 - math.log(lam * s)) / s**2
+        s -= r / dr
+    return mu * (lam - 1.0 / lam) + lbd * math.log(lam * s) / lam, s
+
+
+def _stretch_plate(n=4, t=0.3):
+    nodes, el = O.rect_quad(n, n, 1.0, 1.0)
+    m = O.OracleModel(nodes, el, O.Quad1, "planar",
+                      O.plane_stress_C(torch.full((len(el),), 1.0), torch.tensor(0.3)),
+                      thickness=torch.full((len(el),), t, dtype=DT))
+    con = torch.zeros(len(nodes), 2, dtype=torch.bool)
+    con[nodes[:, 0] == 0.0, 0] = True
+    con[nodes[:, 1] == 0.0, 1] = True
+    right = nodes[:, 0] == 1.0
+    con[right, 0] = True
+    return nodes, el, m, con, right
+
+
+@pytest.mark.parametrize("kind", ["stress", "strain"])
+def test_planar_hyperelastic_uniaxial_stretch(kind):
+    """A Neo-Hookean plate stretched to 1.8x in four increments: in plane
+    stress the lateral and out-of-plane stretches are equal and the reaction
+    is the 3-D uniaxial closed form; in plane strain the lateral stretch
+    solves P_yy = 0 with F_zz = 1. Reactions per unit reference width times
+    the thickness."""
+    mu, lbd, t = 350.0, 800.0, 0.3
+    nodes, el, m, con, right = _stretch_plate(t=t)
+    D = torch.zeros(len(nodes), 2, dtype=DT)
+    D[right, 0] = 0.8
+    params = torch.tensor([mu, lbd], dtype=DT).expand(len(el), 2)
+    step = (O.hyper_plane_stress(O.neo_hookean_psi, params) if kind == "stress"
+            else O.hyper_plane_strain(O.neo_hookean_psi, params))
+    u, f, _, _, state = O.oracle_solve_incremental(
+        m, step, 1 if kind == "stress" else 0, con, torch.zeros_like(D), D,
+        torch.tensor([0.0, 0.25, 0.5, 0.75, 1.0]), nlgeom=True)
+    if kind == "stress":
+        R, _ = uniaxial_stretch_closed_form(mu, lbd, 1.8)
+        lat = None
+    else:
+        R, lat = plane_strain_uniaxial_closed_form(mu, lbd, 1.8)
+    assert abs(float(f[right, 0].sum()) - R * t) < 1e-9 * R * t
+    if lat is not None:
+        assert torch.allclose(u[:, 1], (lat - 1.0) * nodes[:, 1], atol=1e-11)
+    else:
+        # out-of-plane stretch equals the lateral one (isotropic material)
+        s_lat = 1.0 + float(u[nodes[:, 1] == 1.0, 1].mean())
+        assert abs(float(state.mean()) + 1.0 - s_lat) < 1e-10

@@ -29,6 +29,8 @@ __all__ = [
     "IsotropicConductivity3D",
     "IsotropicConductivity2D",
     "Hyperelastic3D",
+    "HyperelasticPlaneStrain",
+    "HyperelasticPlaneStress",
     "IsotropicPlasticity3D",
 ]
 
@@ -276,6 +278,59 @@ class Hyperelastic3D(Material):
             "a hyperelastic material has no constant tangent: its models are solved "
             "by the finite-strain Newton path of FEM.solve"
         )
+
+
+def _embed(F2: Tensor, f33) -> Tensor:
+    """diag(F2, f33) as a 3-D deformation gradient [..., 3, 3]."""
+    F3 = torch.zeros(*F2.shape[:-2], 3, 3, dtype=F2.dtype, device=F2.device)
+    F3[..., :2, :2] = F2
+    F3[..., 2, 2] = f33
+    return F3
+
+
+class HyperelasticPlaneStrain(Hyperelastic3D):
+    """Plane-strain hyperelasticity (hyperelasticity.py:272-330): the 3-D
+    energy at F = diag(F2, 1); P and the tangent are the in-plane blocks."""
+
+    dim = 2
+
+    def stress(self, F: Tensor, params: Tensor | None = None) -> Tensor:
+        return super().stress(_embed(F, 1.0), params)[..., :2, :2]
+
+    def stress_tangent(self, F: Tensor, params: Tensor | None = None):
+        P, A = super().stress_tangent(_embed(F, 1.0), params)
+        return P[..., :2, :2], A[..., :2, :2, :2, :2]
+
+
+class HyperelasticPlaneStress(Hyperelastic3D):
+    """Plane-stress hyperelasticity (hyperelasticity.py:130-269): the
+    out-of-plane stretch lam3 (state: lam3 - 1) solves P33 = 0 by a local
+    Newton iteration from the previous increment's value, and the in-plane
+    tangent is condensed, A_ijkl - A_ij33 A_33kl / A_3333. It carries an
+    internal state, so the incremental path runs its `step`."""
+
+    dim = 2
+    finite_strain = False
+
+    def __init__(self, psi, params, rho: Tensor | float = 1.0, tolerance: float = 1e-5,
+                 max_iter: int = 10):
+        super().__init__(psi, params, rho)
+        self.tolerance = tolerance
+        self.max_iter = max_iter
+        self.n_state = 1
+
+    def step(self, H_inc, F, stress, state, de0, cl, iter):
+        F2 = F + H_inc
+        lam3 = 1.0 + state[..., 0]
+        for _ in range(self.max_iter):
+            P, A = Hyperelastic3D.stress_tangent(self, _embed(F2, lam3))
+            res = P[..., 2, 2]
+            lam3 = lam3 - res / A[..., 2, 2, 2, 2]
+            if not bool((res.abs() >= self.tolerance).any()):
+                break
+        cond = torch.einsum("...ij,...kl->...ijkl", A[..., :2, :2, 2, 2], A[..., 2, 2, :2, :2])
+        A2 = A[..., :2, :2, :2, :2] - cond / A[..., 2, 2, 2, 2][..., None, None, None, None]
+        return P[..., :2, :2], (lam3 - 1.0)[..., None], A2
 
 
 class IsotropicPlasticity3D(IsotropicElasticity3D):

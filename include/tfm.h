@@ -311,8 +311,13 @@ int tfm_pcg_destroy(tfm_pcg_t *h);
  * scipy's cg/minres return for ||b|| = 0).
  * iters[nrhs] receives the total inner iterations, relres[nrhs] the final FP64
  * relative residual, outer[0] the refinement steps (mixed mode).
- * Returns TFM_ERR_NOCONV when max_iter is exhausted, or when the true residual
- * of a mixed solve stops falling above rtol (attainable accuracy reached). */
+ * A mixed solve whose FP32 iterations break down (p.Kp <= 0 in FP32, or a
+ * true residual that stops falling above rtol) continues in FP64 from the x of
+ * its last refinement when sys->val64 is given; iters then counts both parts
+ * (tfm_pcg_fp64_fallback returns the FP64 part).
+ * Returns TFM_ERR_NOCONV when max_iter is exhausted, when p.Kp <= 0 in FP64
+ * (K is not positive definite), or when the true residual stops falling above
+ * rtol (attainable accuracy reached). */
 int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t s, const tfm_system_t *sys,
                   int32_t prec, int32_t pc, const double *b, double *x,
                   int32_t nrhs, double rtol, double inner_rtol, double final_rtol,
@@ -325,6 +330,12 @@ int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t s, const tfm_system_t *sys,
  * Measurement plumbing with no counterpart in the reference. */
 int tfm_pcg_timing(tfm_pcg_t *h, int32_t enable, double *spmv_ms,
                    int64_t *spmv_launches);
+
+/* FP64 iterations that finished the last tfm_pcg_solve after its FP32
+ * iterations broke down (the maximum over its right-hand sides; 0 when the
+ * FP32 iterations sufficed or the solve ran in FP64). No counterpart in the
+ * reference, which solves in FP64 only. */
+int tfm_pcg_fp64_fallback(const tfm_pcg_t *h, int32_t *fp64_iters);
 
 /* Preconditioned CG for a general SPD CSR matrix (FP64, Jacobi). Like
  * tfm_pcg_solve it fails with TFM_ERR_NOCONV as soon as an iteration finds

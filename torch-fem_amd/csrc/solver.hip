@@ -1344,6 +1344,10 @@ struct tfm_pcg {
   double *h_res = nullptr;
   double *h_scal = nullptr;                                // a batch's scalars
   // optional event timing of the SpMV launches (kernel A) of the iterations
+  // the FP32 arithmetic of the last mixed solve broke down (p.Kp <= 0, or a
+  // true residual that stopped falling), and the FP64 iterations that finished it
+  int breakdown = 0;
+  int32_t fallback_iters = 0;
   int timing = 0;  // 0: off; k: event pair around the SpMV of every k-th iteration
   double spmv_ms = 0.0;
   int64_t spmv_count = 0;
@@ -1444,6 +1448,12 @@ extern "C" int tfm_pcg_timing(tfm_pcg_t *h, int32_t enable, double *spmv_ms,
   return TFM_OK;
 }
 
+extern "C" int tfm_pcg_fp64_fallback(const tfm_pcg_t *h, int32_t *fp64_iters) {
+  TFM_REQUIRE(h != nullptr && fp64_iters != nullptr, "null argument");
+  *fp64_iters = h->fallback_iters;
+  return TFM_OK;
+}
+
 namespace {
 
 // Operator callbacks of one solve: kernel A of an iteration (reads n_in
@@ -1495,6 +1505,7 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
   std::vector<double> htol(NRP, -1.0), bnorm(NRP, 0.0), rprev(NRP, 0.0), rnow(NRP, 0.0);
   std::vector<int> fin(NRP, 1), stall(NRP, 0);
   for (int r = 0; r < nrhs; ++r) iters[r] = 0;
+  h->breakdown = 0;
 
   // true residual of rhs r at the current x, staged as the inner residual column
   auto stage = [&](int r, bool first) -> int {
@@ -1613,6 +1624,11 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
         const double eta = h->h_scal[((size_t)(i - start) * NRP + r) * NSCAL + S_PQ];
         if (!(eta > 0.0)) {
           set_error("the matrix is not positive definite (p.Kp <= 0 in CG)");
+          for (int q = 0; q < nrhs; ++q)  // iterations of the unfinished segments
+            if ((running >> q) & 1)
+              iters[q] += std::max(0, (h->h_done[q] >= 0 ? std::min(i, h->h_done[q]) : i) - segf[q]);
+          if (outer) *outer = *std::max_element(nseg.begin(), nseg.end());
+          h->breakdown = 1;
           return TFM_ERR_NOCONV;
         }
       }
@@ -1668,6 +1684,7 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
       if (!fin[r] && stall[r] >= 8) {
         set_error("PCG stagnated above the tolerance (attainable accuracy reached)");
         if (outer) *outer = *std::max_element(nseg.begin(), nseg.end());
+        h->breakdown = 1;
         return TFM_ERR_NOCONV;
       }
     if (mask) {
@@ -1790,10 +1807,11 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
   auto residual = [&](const double *bb, const double *xx, double *res, double *rr, double *b2) -> int {
     return fem_residual(h, s, *sys, bb, xx, res, rr, b2, extended);
   };
+  h->fallback_iters = 0;
   TFM_ND_SWITCH(m.ndof, TFM_NRP_SWITCH(nrp, {
-    if (prec == TFM_PREC_FP64) {
-      rc = build_precond<double>(s, m.n_nod, ND, sys->diag_blocks, pc, (double *)h->minv);
-      if (rc != TFM_OK) return rc;
+    auto run_fp64 = [&](int32_t *its, int32_t *out) -> int {
+      int rc64 = build_precond<double>(s, m.n_nod, ND, sys->diag_blocks, pc, (double *)h->minv);
+      if (rc64 != TFM_OK) return rc64;
       Ops<double> ops;
       ops.scal = spmv_shape<double>(m, NRP).scal;
       ops.spmv_dir = [&](const double *u, double *p, double *sv, int it,
@@ -1805,8 +1823,11 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
         return spmv_launch<double, ND, NRP>(s, m, sys->val64, xi, yo);
       };
       ops.residual = residual;
-      rc = run_pcg<double, ND, NRP>(h, s, m.n_nod, ops, b, x, nrhs, rtol, inner_rtol, final_rtol,
-                                    iters, relres, outer);
+      return run_pcg<double, ND, NRP>(h, s, m.n_nod, ops, b, x, nrhs, rtol, inner_rtol,
+                                      final_rtol, its, relres, out);
+    };
+    if (prec == TFM_PREC_FP64) {
+      rc = run_fp64(iters, outer);
     } else {
       rc = build_precond<float>(s, m.n_nod, ND, sys->diag_blocks, pc, (float *)h->minv);
       if (rc != TFM_OK) return rc;
@@ -1823,6 +1844,18 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
       ops.residual = residual;
       rc = run_pcg<float, ND, NRP>(h, s, m.n_nod, ops, b, x, nrhs, rtol, inner_rtol, final_rtol,
                                    iters, relres, outer);
+      // FP32 rounding of a matrix whose element stiffnesses span many orders
+      // of magnitude (void elements of a density design) can make the FP32
+      // operator indefinite or cap the accuracy of its corrections: then
+      // finish in FP64 from the x of the last refinement
+      if (rc == TFM_ERR_NOCONV && h->breakdown && sys->val64) {
+        std::vector<int32_t> its64(nrhs, 0);
+        int32_t out64 = 0;
+        rc = run_fp64(its64.data(), &out64);
+        for (int r = 0; r < nrhs; ++r) iters[r] += its64[r];
+        h->fallback_iters = *std::max_element(its64.begin(), its64.end());
+        if (outer) *outer += out64;
+      }
     }
   }));
   return rc;

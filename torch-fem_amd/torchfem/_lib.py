@@ -148,6 +148,7 @@ _SIGNATURES: dict[str, tuple[list[Any], Any]] = {
         C.c_int,
     ),
     "tfm_pcg_timing": ([P, I32, C.POINTER(F64), C.POINTER(I64)], C.c_int),
+    "tfm_pcg_fp64_fallback": ([P, C.POINTER(I32)], C.c_int),
     "tfm_csr_pcg_solve": (
         [P, P, I64, P, P, P, P, P, F64, C.POINTER(I32), C.POINTER(F64)],
         C.c_int,

@@ -155,10 +155,14 @@ class PCGSolver:
                                    L.ptr(b), L.ptr(x), nrhs, float(rtol), float(inner_rtol),
                                    float(0.7 * rtol if final_rtol is None else final_rtol),
                                    its, rel, C.byref(outer))
+        fb = L.I32()
+        L.check(L.lib().tfm_pcg_fp64_fallback(self._h, C.byref(fb)))
         info = dict(iterations=list(its), relres=list(rel), outer=int(outer.value),
-                    time=time.perf_counter() - t0, precision=precision, preconditioner=preconditioner)
+                    time=time.perf_counter() - t0, precision=precision, preconditioner=preconditioner,
+                    fp64_fallback=int(fb.value))
         if rc == L.TFM_ERR_NOCONV:
-            raise RuntimeError(f"PCG did not converge: {info}")
+            msg = L.lib().tfm_last_error().decode(errors="replace")
+            raise RuntimeError(f"PCG did not converge ({msg}): {info}")
         L.check(rc)
         return info
 

@@ -366,6 +366,299 @@ __global__ void __launch_bounds__(64) k_assemble_rows(tfm_mesh_t m, tfm_material
   }
 }
 
+// Node-tiled assembly. A 64-lane block assembles the rows of NW consecutive
+// nodes. A node owns NPE * IP lanes: IP of its incident elements are taken at
+// once (incidence t = r IP + ip in round r), each by NPE lanes, lane j for the
+// element's column node j. Per incidence, lane g < NGP evaluates the Jacobian
+// at Gauss point g once and shares J^{-1} and the scaled row gradient
+// w_g det J_g s_e dN_i/dx through LDS; lane j then forms the geometric moments
+//   M_ij[a][c] = sum_g w_g det J_g s_e (dN_i/dx)_a (dN_j/dx)_c
+// and the material block once (isotropic elasticity: lam M + G M^T + G tr(M) I;
+// heat: kappa : M), the per-point form of coupling_block summed over the
+// points. The blocks of a round are added into per-node row accumulators in
+// LDS one incidence group after the other (ascending t, a fixed order, so the
+// sums are bitwise reproducible), and the tile's rows, contiguous in the value
+// arrays, leave in one coalesced pass with the Dirichlet epilogue of
+// k_assemble_rows. The element data of RB rounds are loaded together: a batch
+// costs three dependent global round trips.
+// IP and RB per element type, from timings on the 64^3 Hexa1, 80^3 Tetra1 and
+// 32^3 Hexa2 cubes (IP 1-4 and RB 1-4 swept; occupancy is bound by VGPRs and
+// LDS together)
+template <class E> struct NodeTile;
+template <> struct NodeTile<Quad1> { static constexpr int IP = 4, RB = 1; };
+template <> struct NodeTile<Tetra1> { static constexpr int IP = 2, RB = 1; };
+template <> struct NodeTile<Hexa1> { static constexpr int IP = 2, RB = 2; };
+template <> struct NodeTile<Hexa2> { static constexpr int IP = 1, RB = 4; };
+// the general (anisotropic) stiffness has 81 parameters per element, too many
+// to hold per lane: it keeps the one-node-per-block kernel
+template <int MAT> constexpr bool node_tiled() { return MAT != TFM_MAT_GENERAL; }
+template <class E> constexpr int tile_nodes() { return 64 / (E::NPE * NodeTile<E>::IP); }
+
+template <class E, int NDOF, int MAT>
+__global__ void __launch_bounds__(64) k_assemble_nodes(tfm_mesh_t m, tfm_material_t mat,
+                                                       const uint8_t *con, double *v64,
+                                                       float *v32, double *dblk, int32_t *bad,
+                                                       int diff32) {
+  constexpr int NPE = E::NPE, NGP = E::NGP, D = E::DIM, NP = mat_nparam(MAT, D);
+  constexpr int IP = NodeTile<E>::IP, RB = NodeTile<E>::RB;
+  constexpr int ND2 = NDOF * NDOF, LPN = NPE * IP, NW = tile_nodes<E>(), NG = NW * IP;
+  constexpr int GW = D * D + D;  // per point: J^{-1}, then the scaled row gradient
+  constexpr int CQ = 4;          // Dirichlet-bit chunk: 64 CQ columns
+  static_assert(NW >= 1 && NGP <= NPE, "a tile holds whole nodes, one point per lane");
+  static_assert(MAT != TFM_MAT_GENERAL, "the general stiffness keeps k_assemble_rows");
+  extern __shared__ double acc[];  // [NW][ND2 * max_deg] row accumulators, then
+                                   // [NW][max_deg] Dirichlet bits of the columns
+  // row strides padded by two doubles: the 8-byte banks of rows that lanes
+  // read at the same time then differ (no bank conflicts between groups)
+  constexpr int TS = D * NPE + 2, XS = NPE * D + 2, GS = NGP * GW + 2;
+  __shared__ double tbl[NGP][TS];  // dN[g][a][N] at [g][a * NPE + N]
+  __shared__ double tw[NGP];
+  __shared__ double Xs[NG][XS];   // [group][N * D + c]
+  __shared__ double Gm[NG][GS];   // [group][g * GW + ...]
+  __shared__ int64_t s_vb[NW + 1];
+  __shared__ int s_a0[NW], s_deg[NW], s_self[NW], s_crow[NW];
+
+  const int lane = threadIdx.x;
+  const int w = lane / LPN, ip = (lane / NPE) % IP, j = lane % NPE, grp = w * IP + ip;
+  const int64_t n0 = xcd_block() * (int64_t)NW;
+  const int64_t n = n0 + w;
+  const bool valid = lane < NW * LPN && n < m.n_nod;
+  const int stride = ND2 * m.max_deg;
+  unsigned char *cmask = reinterpret_cast<unsigned char *>(acc + NW * stride);
+  int inc0 = 0, ninc = 0;
+  if (valid) {
+    inc0 = m.ne_ptr[n];
+    ninc = m.ne_ptr[n + 1] - inc0;
+  }
+  if (lane < NW) {
+    const int64_t nn = n0 + lane;
+    const bool ok = nn < m.n_nod;
+    s_a0[lane] = ok ? m.adj_ptr[nn] : 0;
+    s_deg[lane] = ok ? m.adj_ptr[nn + 1] - m.adj_ptr[nn] : 0;
+    s_vb[lane] = m.val_ptr[ok ? nn : m.n_nod];
+    s_self[lane] = -1;
+    int cr = 0;
+    for (int a = 0; a < NDOF; ++a) cr |= ok && con[nn * NDOF + a] ? 1 << a : 0;
+    s_crow[lane] = cr;
+    if (lane == NW - 1) s_vb[NW] = m.val_ptr[nn + 1 < m.n_nod ? nn + 1 : m.n_nod];
+  }
+  const auto &T = shape_table<E>();
+  for (int q = lane; q < NGP * D * NPE; q += 64) tbl[q / (D * NPE)][q % (D * NPE)] = (&T.dN[0][0][0])[q];
+  for (int q = lane; q < NGP; q += 64) tw[q] = T.w[q];
+  for (int q = lane; q < NW * stride; q += 64) acc[q] = 0.0;
+  __syncthreads();
+
+  // Dirichlet bits of every column node of the tile's rows (two dependent
+  // loads each), in chunks of 64 CQ columns; the first chunk rides along with
+  // the element loads of the first batch, and the epilogue reads LDS only
+  const int nq = NW * m.max_deg;
+  auto cm_adj = [&](int q0, int (&nbr)[CQ]) {
+#pragma unroll
+    for (int c = 0; c < CQ; ++c) {
+      const int q = q0 + c * 64 + lane;
+      nbr[c] = -1;
+      if (q < nq) {
+        const int ww = q / m.max_deg, p = q - ww * m.max_deg;
+        if (p < s_deg[ww]) nbr[c] = m.adj[s_a0[ww] + p];
+      }
+    }
+  };
+  auto cm_con = [&](int q0, const int (&nbr)[CQ]) {
+#pragma unroll
+    for (int c = 0; c < CQ; ++c) {
+      const int q = q0 + c * 64 + lane;
+      int bits = 0;
+      if (nbr[c] >= 0)
+        for (int b = 0; b < NDOF; ++b) bits |= con[(int64_t)nbr[c] * NDOF + b] ? 1 << b : 0;
+      if (q < nq) cmask[q] = (unsigned char)bits;
+    }
+  };
+
+  const int rounds = (m.max_inc + IP - 1) / IP;
+  for (int r0 = 0; r0 < rounds; r0 += RB) {
+    // the batch's element data: incidences, then connectivity, slots and
+    // parameters, then coordinates
+    int e[RB], il[RB];
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      const int t = (r0 + b) * IP + ip;
+      e[b] = -1;
+      il[b] = 0;
+      if (valid && t < ninc) {
+        e[b] = m.ne_elem[inc0 + t];
+        il[b] = m.ne_loc[inc0 + t];
+      }
+    }
+    int nbr[CQ];
+    if (r0 == 0) cm_adj(0, nbr);
+    int64_t id[RB];
+    int P[RB];  // slot of the lane's column node in the row node's adjacency
+    double mp[RB][NP], se[RB];
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      id[b] = 0;
+      P[b] = 0;
+      se[b] = 0.0;
+      for (int q = 0; q < NP; ++q) mp[b][q] = 0.0;
+      if (e[b] >= 0) {
+        const int64_t eb = e[b];
+        id[b] = m.elements[eb * NPE + j];
+        P[b] = m.epos[(eb * NPE + il[b]) * NPE + j];
+        for (int q = 0; q < NP; ++q) mp[b][q] = mat_param<MAT, D>(mat, eb, q);
+        se[b] = mat_scale(mat, eb);
+      }
+    }
+    if (r0 == 0) cm_con(0, nbr);
+    double X[RB][D];
+#pragma unroll
+    for (int b = 0; b < RB; ++b)
+      for (int d = 0; d < D; ++d) X[b][d] = e[b] >= 0 ? m.nodes[id[b] * D + d] : 0.0;
+
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      if (r0 + b >= rounds) break;
+      const bool act = e[b] >= 0;
+      const int i = il[b];
+      if (act) {
+        for (int d = 0; d < D; ++d) Xs[grp][j * D + d] = X[b][d];
+        if (r0 + b == 0 && ip == 0 && j == i) s_self[w] = P[b];
+      }
+      __syncthreads();
+      if (act && j < NGP) {  // lane g = j: the geometry of point g
+        const int g = j;
+        double J[D][D];
+        for (int a = 0; a < D; ++a)
+          for (int c = 0; c < D; ++c) J[a][c] = 0.0;
+        for (int N = 0; N < NPE; ++N)
+          for (int a = 0; a < D; ++a)
+            for (int c = 0; c < D; ++c) J[a][c] += tbl[g][a * NPE + N] * Xs[grp][N * D + c];
+        double Ji[D][D];
+        const double det = inv_jac<D>(J, Ji);
+        if (!(det > 0.0)) *bad = 1;
+        const double wd = tw[g] * det * se[b];
+        double *G = &Gm[grp][g * GW];
+        for (int a = 0; a < D; ++a) {
+          double v = 0.0;
+          for (int c = 0; c < D; ++c) {
+            G[a * D + c] = Ji[a][c];
+            v += Ji[a][c] * tbl[g][c * NPE + i];
+          }
+          G[D * D + a] = v * wd;
+        }
+      }
+      __syncthreads();
+      double blk[ND2];
+      if (act) {
+        double M[D * D];
+        for (int q = 0; q < D * D; ++q) M[q] = 0.0;
+        for (int g = 0; g < NGP; ++g) {
+          const double *G = &Gm[grp][g * GW];
+          double bj[D];
+          for (int a = 0; a < D; ++a) {
+            double v = 0.0;
+            for (int c = 0; c < D; ++c) v += G[a * D + c] * tbl[g][c * NPE + j];
+            bj[a] = v;
+          }
+          if constexpr (MAT == TFM_MAT_HEAT_ISO) {
+            for (int a = 0; a < D; ++a) M[0] += G[D * D + a] * bj[a];
+          } else {
+            for (int a = 0; a < D; ++a)
+              for (int c = 0; c < D; ++c) M[a * D + c] += G[D * D + a] * bj[c];
+          }
+        }
+        if constexpr (MAT == TFM_MAT_ISO) {
+          double tr = 0.0;
+          for (int a = 0; a < D; ++a) tr += M[a * D + a];
+          for (int a = 0; a < D; ++a)
+            for (int c = 0; c < D; ++c)
+              blk[a * NDOF + c] = mp[b][0] * M[a * D + c] + mp[b][1] * M[c * D + a] +
+                                  (a == c ? mp[b][1] * tr : 0.0);
+        } else if constexpr (MAT == TFM_MAT_HEAT_ISO) {
+          blk[0] = mp[b][0] * M[0];
+        } else {
+          double v = 0.0;
+          for (int q = 0; q < D * D; ++q) v += mp[b][q] * M[q];
+          blk[0] = v;
+        }
+      }
+      // add the blocks in ascending incidence order: group ip at step ip
+      for (int st = 0; st < IP; ++st) {
+        if (act && ip == st) {
+          double *A = acc + w * stride;
+          const int L = NDOF * s_deg[w];
+          for (int a = 0; a < NDOF; ++a)
+            for (int c = 0; c < NDOF; ++c) A[a * L + P[b] * NDOF + c] += blk[a * NDOF + c];
+        }
+        __syncthreads();
+      }
+    }
+  }
+  for (int q0 = rounds > 0 ? 64 * CQ : 0; q0 < nq; q0 += 64 * CQ) {
+    int nbr[CQ];
+    cm_adj(q0, nbr);
+    cm_con(q0, nbr);
+  }
+  __syncthreads();
+  // the tile's rows are contiguous in the value arrays: [s_vb[0], s_vb[NW])
+  const int64_t vb0 = s_vb[0];
+  const int total = (int)(s_vb[NW] - vb0);
+  int wq = 0, beg = 0, end = (int)(s_vb[1] - vb0);  // lane's node and its value range
+  for (int q = lane; q < total; q += 64) {
+    while (end <= q) {
+      ++wq;
+      beg = end;
+      end = (int)(s_vb[wq + 1] - vb0);
+    }
+    const int64_t nn = n0 + wq;
+    const int deg = s_deg[wq];
+    const int ql = q - beg;
+    if (deg == 0) {  // orphan node: only its diagonal is stored (base.py:89-90)
+      const double v = (s_crow[wq] >> ql) & 1 ? 1.0 : 0.0;
+      if (v64) v64[vb0 + q] = v;
+      if (v32) v32[vb0 + q] = (float)v;
+      continue;
+    }
+    const int L = NDOF * deg;
+    int a = 0, rem = ql;  // ql = a L + rem
+    for (int k = 1; k < NDOF; ++k)
+      if (rem >= L) {
+        rem -= L;
+        ++a;
+      }
+    const int p = rem / NDOF, b = rem - p * NDOF;
+    const int self = s_self[wq];
+    double *A = acc + wq * stride;
+    double v = A[ql];
+    if (((s_crow[wq] >> a) & 1) || ((cmask[wq * m.max_deg + p] >> b) & 1))
+      v = (p == self && a == b) ? 1.0 : 0.0;  // row == col
+    A[ql] = v;
+    if (v64) v64[vb0 + q] = v;
+    if (v32 && (p != self || !diff32)) v32[vb0 + q] = (float)v;
+    if (p == self) dblk[nn * ND2 + a * NDOF + b] = v;
+  }
+  for (int q = lane; q < NW * ND2; q += 64) {  // orphan diagonal blocks
+    const int ww = q / ND2, ab = q - ww * ND2, a = ab / NDOF, b = ab - a * NDOF;
+    const int64_t nn = n0 + ww;
+    if (nn < m.n_nod && s_deg[ww] == 0)
+      dblk[nn * ND2 + ab] = (a == b && ((s_crow[ww] >> a) & 1)) ? 1.0 : 0.0;
+  }
+  // FP32 difference form: the diagonal block holds the row sums of the
+  // final FP64 values, added in ascending slot order
+  if (v32 && diff32) {
+    __syncthreads();
+    for (int q = lane; q < NW * ND2; q += 64) {
+      const int ww = q / ND2, ab = q - ww * ND2, a = ab / NDOF, b = ab - a * NDOF;
+      const int deg = s_deg[ww];
+      if (n0 + ww >= m.n_nod || deg == 0) continue;
+      const int L = NDOF * deg;
+      const double *A = acc + ww * stride;
+      double r = 0.0;
+      for (int p = 0; p < deg; ++p) r += A[a * L + p * NDOF + b];
+      v32[s_vb[ww] + a * L + s_self[ww] * NDOF + b] = (float)r;
+    }
+  }
+}
+
 // =========================================================== element teams
 // Kernels below give each element a team of LG = next_pow2(NPE) lanes, one
 // lane per element node; Jacobians and gradients are summed across the team
@@ -741,7 +1034,20 @@ int launch_assemble(hipStream_t s, const tfm_mesh_t &m, const tfm_material_t &ma
                     const uint8_t *con, double *v64, float *v32, double *dblk, int32_t *bad,
                     bool diff32) {
   if (m.n_nod == 0) return TFM_OK;
+  const int rows_only = env_int("TFM_ASSEMBLE_ROWS", 0);
   TFM_DISPATCH_ALL(m, mat, {
+    constexpr int NW = tile_nodes<ET>();
+    const size_t tile_shm = (sizeof(double) * ND * ND + 1) * NW * (size_t)m.max_deg;
+    if (node_tiled<MK>() && !rows_only && tile_shm <= 48 * 1024) {
+      constexpr int MT = node_tiled<MK>() ? MK : TFM_MAT_ISO;
+      const int64_t nb = (m.n_nod + NW - 1) / NW;
+      hipLaunchKernelGGL((k_assemble_nodes<ET, ND, MT>), dim3((unsigned)nb), dim3(64), tile_shm,
+                         s, m, mat, con, v64, v32, dblk, bad, (int)diff32);
+      TFM_CHECK_LAUNCH();
+      return TFM_OK;
+    }
+    // one node per block: the general stiffness, node degrees whose tile would
+    // not fit in LDS, or TFM_ASSEMBLE_ROWS=1
     constexpr int KCH = assemble_chunk<ET>();
     const size_t shm = sizeof(double) * ND * ND * (size_t)m.max_deg + (size_t)KCH * m.max_deg;
     TFM_REQUIRE(shm <= 48 * 1024, "node degree too large for the assembly kernel");

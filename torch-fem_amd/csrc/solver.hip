@@ -1118,11 +1118,6 @@ __global__ void __launch_bounds__(1024) k_sum2(const double *part, int nblk, dou
 }
 
 // ------------------------------------------------------- launch helpers
-static int env_int(const char *name, int dflt) {
-  const char *v = std::getenv(name);
-  return (v && *v) ? std::atoi(v) : dflt;
-}
-
 static unsigned grid_for(int64_t work, int per_block, int cap) {
   const int64_t g = (work + per_block - 1) / per_block;
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, cap));

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <string>
 
@@ -158,6 +159,12 @@ template <class E> constexpr ShapeTable<E> make_table() {
 }
 
 template <class E> __device__ const ShapeTable<E> &shape_table();
+
+// integer tuning knob from the environment (dflt when unset or empty)
+inline int env_int(const char *name, int dflt) {
+  const char *v = std::getenv(name);
+  return (v && *v) ? std::atoi(v) : dflt;
+}
 
 // Logical index of this block for one-block-per-item kernels: workgroups are
 // dispatched to the 8 XCDs round-robin (block b on XCD b % 8), so the blocks

@@ -452,10 +452,16 @@ class FEM(ABC):
         if perm is not None:  # to the solver's node order
             b, x = b[:, perm].contiguous(), x[:, perm].contiguous()
         solver = self._solver()
-        info = solver.solve_system(system.sys, b, x, precision=cfg["precision"],
+        # once the FP32 iterations of a mixed solve broke down on this system
+        # (its FP32 operator is not positive definite), later solves with the
+        # same matrix (the adjoint, further Newton steps) run in FP64 directly
+        precision = "fp64" if getattr(system, "fp32_broke", False) else cfg["precision"]
+        info = solver.solve_system(system.sys, b, x, precision=precision,
                                    preconditioner=cfg["pc"], rtol=cfg["stol"],
                                    inner_rtol=cfg["inner_rtol"],
                                    final_rtol=(0.7 if adjoint else 0.25) * cfg["stol"])
+        if info.get("fp64_fallback", 0) > 0:
+            system.fp32_broke = True
         self._last_pcg = info
         if perm is not None:
             out = torch.empty_like(x)

@@ -185,6 +185,40 @@ def test_tiled_and_per_node_assembly_agree(name, monkeypatch):
     assert float((tiled.val32.double() - rows.val32.double()).abs().max()) < 2e-7 * scale
 
 
+def test_tiled_assembly_with_orphan_nodes(monkeypatch):
+    """Nodes of no element keep only their diagonal (base.py:89-90): 1 on a
+    constrained DOF, 0 otherwise, in both assembly kernels and the oracle.
+    The orphans sit in the middle of the numbering, inside a tile."""
+    nodes, el = O.cube_hexa(4, 3, 3)
+    n0 = len(nodes)
+    extra = torch.tensor([[5.0, 5.0, 5.0], [6.0, 6.0, 6.0]], dtype=DT)
+    # renumber so that the two orphans become nodes 5 and 6
+    perm = torch.cat([torch.arange(5), torch.tensor([n0, n0 + 1]), torch.arange(5, n0)])
+    allnodes = torch.cat([nodes, extra])[perm]
+    inv = torch.empty_like(perm)
+    inv[perm] = torch.arange(len(perm))
+    el2 = inv[el]
+    E = 800.0 + 400.0 * torch.rand(len(el2), generator=torch.Generator().manual_seed(2), dtype=DT)
+    m = Solid(allnodes.cuda(), el2.cuda(), IsotropicElasticity3D(E, 0.3))
+    o = O.OracleModel(allnodes, el2, O.Hexa1, "solid", O.iso3d_C(E, torch.tensor(0.3)))
+    con = torch.zeros(len(allnodes), 3, dtype=torch.bool)
+    con[allnodes[:, 0] == 0.0] = True
+    con[5, 1] = True  # one constrained DOF of an orphan
+    m.constraints = con.cuda()
+    kind, p0, p1, sc = m._material_params()
+    tiled = _System(m, kind, p0, p1, sc, "mixed")
+    monkeypatch.setenv("TFM_ASSEMBLE_ROWS", "1")
+    rows = _System(m, kind, p0, p1, sc, "mixed")
+    monkeypatch.delenv("TFM_ASSEMBLE_ROWS")
+    ref = o.assemble_matrix(o.k0(), torch.nonzero(con.ravel()).ravel())
+    for sysm in (tiled, rows):
+        v = sysm.val64.cpu()
+        assert torch.allclose(v, ref, rtol=1e-12, atol=1e-12 * ref.abs().max())
+        D = sysm.diag.cpu().reshape(-1, 3, 3)
+        assert torch.equal(D[5], torch.diag(torch.tensor([0.0, 1.0, 0.0], dtype=DT)))
+        assert torch.equal(D[6], torch.zeros(3, 3, dtype=DT))
+
+
 @pytest.mark.parametrize("name,general", CASES)
 def test_spmv(name, general):
     m, o = _case(name, general)

@@ -46,6 +46,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--inner-rtol", type=float, default=0.1,
                     help="mixed precision: residual drop between reliable updates")
     ap.add_argument("--fp64-steps", type=int, default=5, help="0 skips the FP64 timing")
+    ap.add_argument("--two-level-steps", type=int, default=5,
+                    help="timed mixed steps with preconditioner='two_level' beside the "
+                         "block-Jacobi headline (0 skips them)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-stride", type=int, default=32,
                     help="HIP-event timing of the SpMV kernel on every k-th PCG iteration "
@@ -533,6 +536,25 @@ def main() -> None:
         ms_fp64, (u64, gE64, _, _, _) = timed(step_fp64, args.fp64_steps, 1, "fp64")
         info_fp64 = dict(model.last_solve_info)
 
+    # the same step with the two-level preconditioner (coarse correction on
+    # node aggregates), reported beside the block-Jacobi headline
+    two, u_two, gE_two = None, None, None
+    if args.two_level_steps > 0 and args.preconditioner != "two_level":
+        step_two = make_step(model, "mixed", args.stol, "two_level", args.inner_rtol)
+        ms_two, (u_two, gE_two, _, _, _) = timed(step_two, args.two_level_steps, 1, "two-level")
+        info_two = dict(model.last_solve_info)
+        two = {
+            "preconditioner": "two_level (block-Jacobi + coarse correction on node aggregates)",
+            "ms_per_step": ms_two * 1e3,
+            "value": model.n_dofs / ms_two,
+            "unit": "DOFs/s",
+            "speedup_vs_block_jacobi": ms_mixed / ms_two,
+            "pcg_iterations": {
+                "mixed_fwd": sum(p["iterations"][0] for p in info_two["forward"]["pcg"]),
+                "mixed_adj": info_two["adjoint"]["iterations"][0]},
+        }
+        log(f"two-level {two}")
+
     # FP64 reference of the same problem solved tightly, for the error report
     acc = {
         "relres_forward": info_mixed["forward"]["pcg"][-1]["relres"][0],
@@ -553,6 +575,9 @@ def main() -> None:
         if u64 is not None:  # the FP64 path at the same tolerance, for scale
             acc["fp64_same_stol_u_rel_err"] = rel(u64, u_ref)
             acc["fp64_same_stol_gradE_rel_err"] = rel(gE64, gE_ref)
+        if two is not None:
+            two["u_rel_err_vs_fp64"] = rel(u_two, u_ref)
+            two["gradE_rel_err_vs_fp64"] = rel(gE_two, gE_ref)
     log(f"accuracy {acc}")
 
     # roofline of the dominant kernel: the FP32 node-block SpMV fused with the
@@ -614,6 +639,7 @@ def main() -> None:
         "phase_s": {"mixed_fwd_pcg": info_mixed["forward"]["pcg"][-1]["time"],
                     "mixed_adj_pcg": info_mixed["adjoint"]["time"]},
         "accuracy": acc,
+        "two_level": two,
         "roofline": roof,
         "cpu_baseline": None,
         "published_context": {

@@ -80,8 +80,11 @@ enum {
 
 /* Preconditioners. */
 enum {
-  TFM_PC_JACOBI = 0,      /* inverse of diag(K) (cupy path, sparse.py:406-421) */
-  TFM_PC_BLOCK_JACOBI = 1 /* inverse of the ndof x ndof nodal diagonal blocks  */
+  TFM_PC_JACOBI = 0,       /* inverse of diag(K) (cupy path, sparse.py:406-421) */
+  TFM_PC_BLOCK_JACOBI = 1, /* inverse of the ndof x ndof nodal diagonal blocks  */
+  TFM_PC_TWO_LEVEL = 2     /* block-Jacobi plus an additive coarse correction on
+                              node aggregates (tfm_coarse_*; in the place of the
+                              reference's AMG preconditioner, sparse.py:422-442) */
 };
 
 /* Mesh and node-block sparsity pattern, all arrays on the device.
@@ -139,6 +142,7 @@ typedef struct tfm_system {
 typedef struct tfm_pattern tfm_pattern_t;
 typedef struct tfm_pcg tfm_pcg_t;
 typedef struct tfm_krylov tfm_krylov_t;
+typedef struct tfm_coarse tfm_coarse_t;
 
 /* ---------------------------------------------------------------- errors */
 const char *tfm_last_error(void);
@@ -336,6 +340,44 @@ int tfm_pcg_timing(tfm_pcg_t *h, int32_t enable, double *spmv_ms,
  * FP32 iterations sufficed or the solve ran in FP64). No counterpart in the
  * reference, which solves in FP64 only. */
 int tfm_pcg_fp64_fallback(const tfm_pcg_t *h, int32_t *fp64_iters);
+
+/* ------------------------------------------- two-level preconditioner */
+/* Coarse space of TFM_PC_TWO_LEVEL, in the place of the aggregation AMG the
+ * reference reaches through AmgX (sparse.py:422-442, amgx.py:71-98): nodes
+ * are grouped into n_agg aggregates and every aggregate carries the
+ * rigid-body modes of its nodes about its centroid (3 translations and 3
+ * rotations in 3-D, 2 + 1 in 2-D, the constant for a scalar field), zero on
+ * constrained DOFs. The preconditioner is
+ *   M^{-1} r = B^{-1} r + P (P^T K P)^{-1} P^T r
+ * with B the nodal diagonal blocks and P the modes.
+ * Device arrays of the caller (copied): agg [n_nod] the aggregate of each
+ * node; agg_ptr [n_agg+1] / agg_nodes [n_nod] the nodes of each aggregate,
+ * ascending; centroids [n_agg*dim]; slot_agg [n_agg*27] the aggregates
+ * J that K couples to aggregate I, one per slot (-1: none); pair_off
+ * [n_agg*27+1] / pair_node, pair_pos [n_pairs] the node blocks (i, p) (node
+ * i, position p in its adjacency) of every (I, slot), in ascending (i, p).
+ * Create once per mesh and aggregation; the mesh's arrays must outlive the
+ * handle. */
+int tfm_coarse_create(tfm_coarse_t **h, const tfm_mesh_t *m, int32_t n_agg,
+                      const int32_t *agg, const int32_t *agg_ptr,
+                      const int32_t *agg_nodes, const double *centroids,
+                      const int32_t *slot_agg, const int32_t *pair_off,
+                      int64_t n_pairs, const int32_t *pair_node,
+                      const int32_t *pair_pos);
+int tfm_coarse_destroy(tfm_coarse_t *h);
+/* Coarse size n_c = n_agg * modes. */
+int tfm_coarse_size(const tfm_coarse_t *h, int32_t *n_c);
+/* Ac [n_c*n_c] = P^T K P, dense, from the system's FP64 values (sys->val64)
+ * and Dirichlet mask; deterministic. */
+int tfm_coarse_operator(tfm_stream_t s, const tfm_coarse_t *h, const tfm_system_t *sys,
+                        double *Ac);
+/* The caller's inverse of Ac (FP64, and FP32 for mixed-precision solves),
+ * row-major n_c x n_c, read by every preconditioner application; the
+ * buffers must stay valid while solves use them. */
+int tfm_coarse_set_inverse(tfm_coarse_t *h, const double *inv64, const float *inv32);
+/* Attaches the coarse space to a solver handle for TFM_PC_TWO_LEVEL (NULL
+ * detaches). */
+int tfm_pcg_set_coarse(tfm_pcg_t *p, tfm_coarse_t *c);
 
 /* Preconditioned CG for a general SPD CSR matrix (FP64, Jacobi). Like
  * tfm_pcg_solve it fails with TFM_ERR_NOCONV as soon as an iteration finds

@@ -1,0 +1,186 @@
+"""The two-level preconditioner (preconditioner="two_level": block-Jacobi plus
+an additive coarse correction on node aggregates with their rigid-body
+modes, in the place of the reference's AMG preconditioner,
+sparse.py:422-442). Results against the oracle's direct solves for every
+element type and both precisions, gradients against the SIMP closed form,
+batched load cases, aggregates whose DOFs are all constrained, the solver
+node order of Hexa2 meshes and the iteration counts it saves. Void density
+designs are covered with the other preconditioners in test_gpu_contrast.py."""
+
+import pytest
+import torch
+
+import oracle as O
+from torchfem import Planar, Solid, SolidHeat
+from torchfem.materials import (IsotropicConductivity3D, IsotropicElasticity3D,
+                                IsotropicElasticityPlaneStress)
+from torchfem.mesh import cube_hexa, cube_tetra6, rect_quad
+
+pytestmark = pytest.mark.gpu
+DT = torch.float64
+PRECISIONS = ["fp64", "mixed"]
+
+
+def rel(a, b):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    return float((a - b).norm() / max(float(b.norm()), 1e-300))
+
+
+def _hexa(n, quadratic=False, seed=0):
+    nodes, el = O.cube_hexa(n + 1, n + 1, n + 1)
+    if quadratic:
+        nodes, el = O.linear_to_quadratic(nodes, el)
+    inner = ((nodes > 0) & (nodes < 1)).all(dim=1)
+    nodes = nodes + 0.02 * torch.sin(5.0 * nodes[:, [1, 2, 0]]) * inner[:, None]
+    g = torch.Generator().manual_seed(seed)
+    E = 500.0 + 1000.0 * torch.rand(len(el), generator=g, dtype=DT)
+    return nodes, el, E
+
+
+def _solid_case(n, quadratic=False):
+    nodes, el, E = _hexa(n, quadratic)
+    m = Solid(nodes.cuda(), el.cuda(), IsotropicElasticity3D(E.cuda(), 0.3))
+    X = m.nodes
+    m.constraints[X[:, 0] == 0.0] = True
+    face = X[:, 0] == 1.0
+    m.forces[face, 2] = -1.0 / int(face.sum())
+    m.forces[face, 1] = 0.3 / int(face.sum())
+    o = O.OracleModel(nodes, el, O.Hexa2 if quadratic else O.Hexa1, "solid",
+                      O.iso3d_C(E, torch.tensor(0.3)))
+    return m, o
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("quadratic", [False, True])
+def test_solid_matches_oracle(precision, quadratic):
+    """Distorted Hexa1 (10^3) and Hexa2 (5^3) cubes with random moduli: u
+    and reactions against the oracle."""
+    m, o = _solid_case(5 if quadratic else 10, quadratic)
+    u, f, _, _, _ = m.solve(precision=precision, preconditioner="two_level", stol=1e-11)
+    uo, fo, _, _ = o.solve(m.constraints.cpu(), m.forces.cpu(), m.displacements.cpu())
+    assert rel(u, uo) < 1e-9
+    assert rel(f, fo) < 1e-8
+
+
+def test_solver_node_order():
+    """A 20^3 Hexa2 cube, whose mesh order does not fit 16-bit neighbour
+    offsets, is solved in the locality node order: the coarse space is built
+    on that pattern; against block-Jacobi in the same order."""
+    nodes, el, E = _hexa(20, quadratic=True)
+    m = Solid(nodes.cuda(), el.cuda(), IsotropicElasticity3D(E.cuda(), 0.3))
+    assert m._solver_view() is not None
+    m.constraints[m.nodes[:, 0] == 0.0] = True
+    face = m.nodes[:, 0] == 1.0
+    m.forces[face, 2] = -1.0 / int(face.sum())
+    u, _, _, _, _ = m.solve(precision="mixed", preconditioner="two_level", stol=1e-10)
+    ub, _, _, _, _ = m.solve(precision="fp64", preconditioner="block_jacobi", stol=1e-12)
+    assert rel(u, ub) < 1e-8
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_heat_and_planar_match_oracle(precision):
+    """A scalar field (Tetra1 conduction: one constant mode per aggregate)
+    and a plane-stress Quad1 cantilever (two translations and a rotation)."""
+    nodes, el = O.cube_tetra6(7, jitter=0.1, seed=2)
+    g = torch.Generator().manual_seed(3)
+    k = torch.exp(0.4 * torch.randn(len(el), generator=g, dtype=DT))
+    h = SolidHeat(nodes.cuda(), el.cuda(), IsotropicConductivity3D(k.cuda()))
+    h.constraints[h.nodes[:, 0] == 0.0] = True
+    h.heat_flux = torch.full_like(h.heat_flux, 1e-2)
+    T, _, _, _, _ = h.solve(precision=precision, preconditioner="two_level", stol=1e-11)
+    oh = O.OracleModel(nodes, el, O.Tetra1, "heat", O.iso_KAPPA(k))
+    To, _, _, _ = oh.solve(h.constraints.cpu(), h.heat_flux.cpu(), h.temperatures.cpu())
+    assert rel(T, To) < 1e-9
+
+    nodes, el = O.rect_quad(41, 21, 2.0, 1.0)
+    P = Planar(nodes.cuda(), el.cuda(), IsotropicElasticityPlaneStress(1000.0, 0.3))
+    P.constraints[P.nodes[:, 0] == 0.0] = True
+    tip = P.nodes[:, 0] == 2.0
+    P.forces[tip, 1] = -1.0 / int(tip.sum())
+    u, _, _, _, _ = P.solve(precision=precision, preconditioner="two_level", stol=1e-11)
+    op = O.OracleModel(nodes, el, O.Quad1, "planar",
+                       O.plane_stress_C(torch.full((len(el),), 1000.0, dtype=DT),
+                                        torch.tensor(0.3)),
+                       thickness=torch.ones(len(el), dtype=DT))
+    uo, _, _, _ = op.solve(P.constraints.cpu(), P.forces.cpu(), P.displacements.cpu())
+    assert rel(u, uo) < 1e-9
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_simp_gradient_matches_closed_form(precision):
+    """dC/drho through the adjoint solved with the two-level preconditioner,
+    against -3 rho^2 u_e^T k_e u_e from the oracle's solve (8^3)."""
+    n = 8
+    nodes, el = O.cube_hexa(n + 1, n + 1, n + 1)
+    rho0 = 0.3 + 0.7 * torch.rand(len(el), generator=torch.Generator().manual_seed(0), dtype=DT)
+    rho = rho0.clone().cuda().requires_grad_(True)
+    m = Solid(nodes.cuda(), el.cuda(), IsotropicElasticity3D(1000.0 * rho**3, 0.3))
+    m.constraints[m.nodes[:, 0] == 0.0] = True
+    right = m.nodes[:, 0] == 1.0
+    m.forces[right, 2] = -1.0 / int(right.sum())
+    u, _, _, _, _ = m.solve(differentiable_parameters=rho, precision=precision,
+                            preconditioner="two_level", stol=1e-12)
+    (g,) = torch.autograd.grad(torch.inner(m.forces.ravel(), u.ravel()), rho)
+    o = O.OracleModel(nodes, el, O.Hexa1, "solid", O.iso3d_C(1000.0 * rho0**3, torch.tensor(0.3)))
+    uo, _, _, _ = o.solve(m.constraints.cpu(), m.forces.cpu(), torch.zeros(len(nodes), 3, dtype=DT))
+    unit = O.OracleModel(nodes, el, O.Hexa1, "solid",
+                         O.iso3d_C(torch.ones(len(el), dtype=DT), torch.tensor(0.3)),
+                         pattern=o.pattern)
+    closed = O.simp_compliance_sensitivity(unit, uo, rho0, 1000.0)
+    assert rel(u, uo) < 1e-9
+    assert rel(g, closed) < 1e-8
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_load_cases_and_constrained_aggregates(precision):
+    """Four load cases in one batched solve and adjoint, on a cube clamped
+    over its lower third (whole aggregates have every DOF constrained, so
+    their coarse modes vanish): against one two-level solve per case and
+    against block-Jacobi."""
+    nodes, el = cube_hexa(13, 13, 13)
+    E = torch.full((len(el),), 1000.0, dtype=DT, device="cuda", requires_grad=True)
+    m = Solid(nodes.cuda(), el.cuda(), IsotropicElasticity3D(E, 0.3))
+    X = m.nodes
+    m.constraints[X[:, 2] <= 1.0 / 3.0 + 1e-12] = True
+    top = X[:, 2] == 1.0
+    g = torch.Generator().manual_seed(7)
+    F = torch.zeros(4, m.n_nod, 3, dtype=DT)
+    F[:, top.cpu()] = torch.randn(4, int(top.sum()), 3, generator=g, dtype=DT)
+    F = F.cuda()
+    kw = dict(differentiable_parameters=E, precision=precision, stol=1e-11)
+    U = m.solve_load_cases(F, preconditioner="two_level", **kw)[0]
+    (gE,) = torch.autograd.grad(U.sum(), E)
+    m.material = IsotropicElasticity3D(E, 0.3)  # a fresh graph from E
+    Ub = m.solve_load_cases(F, preconditioner="block_jacobi", **kw)[0]
+    (gEb,) = torch.autograd.grad(Ub.sum(), E)
+    assert rel(U, Ub) < 1e-9
+    assert rel(gE, gEb) < 1e-8
+    m.material = IsotropicElasticity3D(E.detach(), 0.3)
+    for c in (0, 3):
+        m.forces = F[c]
+        u, _, _, _, _ = m.solve(precision=precision, preconditioner="two_level", stol=1e-11)
+        assert rel(U[c], u) < 1e-9
+
+
+def test_iterations_drop():
+    """24^3 Hexa1 compression (the bench's load on a smaller cube): the
+    coarse correction at least halves the PCG iterations of block-Jacobi,
+    forward and adjoint."""
+    nodes, el = cube_hexa(25, 25, 25)
+    E = torch.full((len(el),), 1000.0, dtype=DT, device="cuda", requires_grad=True)
+    m = Solid(nodes.cuda(), el.cuda(), IsotropicElasticity3D(E, 0.3))
+    X = m.nodes
+    m.constraints[X[:, 2] == 0.0, :] = True
+    m.constraints[X[:, 2] == 1.0, 2] = True
+    m.displacements[X[:, 2] == 1.0, 2] = -0.01
+    its = {}
+    for pc in ("block_jacobi", "two_level"):
+        m.material = IsotropicElasticity3D(E, 0.3)
+        u, _, _, _, _ = m.solve(differentiable_parameters=E, precision="fp64", preconditioner=pc,
+                                stol=1e-8)
+        torch.autograd.grad(u.sum(), E)
+        info = m.last_solve_info
+        its[pc] = (info["forward"]["pcg"][0]["iterations"][0], info["adjoint"]["iterations"][0])
+    assert its["two_level"][0] * 2 <= its["block_jacobi"][0], its
+    assert its["two_level"][1] * 2 <= its["block_jacobi"][1], its

@@ -35,6 +35,7 @@ constexpr int RBA = 2048; // max blocks of the SpMV kernels (= eta partials per 
 constexpr int RBV = 512;  // max blocks of the vector kernels whose partials every block of A sums
 constexpr int RBW = 4096; // max blocks of the vector kernels when a scalar kernel sums them
 constexpr int MAXR = 16;  // right-hand sides per solve
+constexpr int CGB = 1;    // partial sets of the coarse correction (after the update's)
 constexpr int NSCAL = 4;  // per-iteration scalars: rz, pq, rr, spare
 constexpr int PCG_CHK_MAX = 64;  // most iterations between two host checks
 enum { S_RZ = 0, S_PQ = 1, S_RR = 2 };
@@ -929,12 +930,12 @@ __global__ void __launch_bounds__(BS) k_pcg_seg_dots(int64_t n_nod, const T *r, 
 template <typename T, int ND, int NRP>
 __global__ void __launch_bounds__(BS) k_pcg_seg_apply(int64_t n_nod, int it, int mask, int cont,
                                                       int running, SegInfo seg, const T *u, T *p,
-                                                      T *s, PcgState st, int nblk) {
+                                                      T *s, PcgState st, int nblk, int nblk_aux) {
   __shared__ double sh[BS];
   __shared__ double tot[2 * NRP], rp[NRP], cz[NRP];
   __shared__ int mode[NRP];
   sum_partials<2 * NRP>(st.part_upd, nblk, sh, tot);
-  sum_partials<NRP>(st.part_aux, nblk, sh, rp);
+  sum_partials<NRP>(st.part_aux, nblk_aux, sh, rp);
   const int tid = threadIdx.x;
   if (tid < NRP) {
     const bool run = (mask >> tid) & 1;
@@ -1308,6 +1309,208 @@ int launch_spmv(hipStream_t s, const tfm_mesh_t &m, int prec, const void *vals, 
   return rc;
 }
 
+// ====================================================================
+// Two-level preconditioner: an additive coarse correction on node
+// aggregates, M^{-1} r = B^{-1} r + P (P^T K P)^{-1} P^T r. The columns of P
+// are the rigid-body modes of each aggregate's nodes about its centroid,
+// zero on constrained DOFs.
+// ====================================================================
+template <int ND> constexpr int coarse_modes() { return ND == 3 ? 6 : ND == 2 ? 3 : 1; }
+
+// Mode values of node i in aggregate I: M[a][k] for DOF a and mode k.
+template <int ND>
+__device__ __forceinline__ void node_modes(const tfm_mesh_t &m, const double *cen,
+                                           const uint8_t *con, int64_t i, int I,
+                                           double (&M)[ND][coarse_modes<ND>()]) {
+  constexpr int NM = coarse_modes<ND>();
+  if constexpr (ND == 1) {
+    M[0][0] = con[i] ? 0.0 : 1.0;
+  } else {
+    double d[ND];
+    for (int a = 0; a < ND; ++a) d[a] = m.nodes[i * ND + a] - cen[(int64_t)I * ND + a];
+    for (int a = 0; a < ND; ++a)
+      for (int k = 0; k < NM; ++k) M[a][k] = (k == a) ? 1.0 : 0.0;
+    if constexpr (ND == 3) {  // rotations e_k x d about the centroid
+      M[1][3] = -d[2]; M[2][3] = d[1];
+      M[0][4] = d[2];  M[2][4] = -d[0];
+      M[0][5] = -d[1]; M[1][5] = d[0];
+    } else {
+      M[0][2] = -d[1]; M[1][2] = d[0];
+    }
+    for (int a = 0; a < ND; ++a)
+      if (con[i * ND + a])
+        for (int k = 0; k < NM; ++k) M[a][k] = 0.0;
+  }
+}
+
+struct CoarseView {
+  int n_agg, n_c;
+  const int32_t *agg, *agg_ptr, *agg_nodes, *slot_agg, *pair_off, *pair_node, *pair_pos;
+  const double *cen;
+  double *rc, *ec;  // [n_c * NRP]
+};
+
+// Ac[(I nm + k), (J nm + l)] for the aggregate pair (I, slot): one wave per
+// (I, s); lane t sums M_i^T K_ij M_j over the node blocks (i, p) = t, t + 64,
+// ... of the pair's list, then the wave adds the lane sums by a butterfly
+// (a fixed order: the result is reproducible).
+template <int ND>
+__global__ void __launch_bounds__(64) k_coarse_operator(tfm_mesh_t m, CoarseView cv,
+                                                        const double *val, const uint8_t *con,
+                                                        double *Ac) {
+  constexpr int NM = coarse_modes<ND>();
+  const int64_t b = blockIdx.x;
+  const int I = (int)(b / 27);
+  const int J = cv.slot_agg[b];
+  if (J < 0) return;
+  const int t = threadIdx.x;
+  double acc[NM][NM];
+  for (int k = 0; k < NM; ++k)
+    for (int l = 0; l < NM; ++l) acc[k][l] = 0.0;
+  for (int q = cv.pair_off[b] + t; q < cv.pair_off[b + 1]; q += 64) {
+    const int64_t i = cv.pair_node[q];
+    const int p = cv.pair_pos[q];
+    const int a0 = m.adj_ptr[i];
+    const int deg = m.adj_ptr[i + 1] - a0;
+    const int64_t j = m.adj[a0 + p];
+    double Mi[ND][NM], Mj[ND][NM];
+    node_modes<ND>(m, cv.cen, con, i, I, Mi);
+    node_modes<ND>(m, cv.cen, con, j, J, Mj);
+    const double *kv = val + m.val_ptr[i] + p * ND;
+    double KM[ND][NM];  // K_ij M_j
+    for (int a = 0; a < ND; ++a)
+      for (int l = 0; l < NM; ++l) KM[a][l] = 0.0;
+    for (int a = 0; a < ND; ++a)
+      for (int c = 0; c < ND; ++c) {
+        const double kac = kv[(int64_t)a * ND * deg + c];
+        for (int l = 0; l < NM; ++l) KM[a][l] += kac * Mj[c][l];
+      }
+    for (int a = 0; a < ND; ++a)
+      for (int k = 0; k < NM; ++k)
+        for (int l = 0; l < NM; ++l) acc[k][l] += Mi[a][k] * KM[a][l];
+  }
+  for (int off = 32; off >= 1; off >>= 1)
+    for (int k = 0; k < NM; ++k)
+      for (int l = 0; l < NM; ++l) acc[k][l] += __shfl_xor(acc[k][l], off, 64);
+  if (t < NM * NM) {
+    const int k = t / NM, l = t % NM;
+    double v = 0.0;
+    for (int kk = 0; kk < NM; ++kk)
+      for (int ll = 0; ll < NM; ++ll)
+        if (kk == k && ll == l) v = acc[kk][ll];
+    Ac[((int64_t)I * NM + k) * cv.n_c + (int64_t)J * NM + l] = v;
+  }
+}
+
+// r_c = P^T r: one block per aggregate; thread t takes rhs t % NRP of every
+// (BS / NRP)-th node of the aggregate, and the block adds the thread sums in
+// thread order.
+template <typename T, int ND, int NRP>
+__global__ void __launch_bounds__(BS) k_coarse_restrict(tfm_mesh_t m, CoarseView cv,
+                                                        const uint8_t *con, const T *r,
+                                                        const int *alldone) {
+  constexpr int NM = coarse_modes<ND>(), CH = BS / NRP;
+  __shared__ double sh[BS * NM];
+  if (*alldone) return;
+  const int I = blockIdx.x, t = threadIdx.x, rr = t % NRP, c = t / NRP;
+  double acc[NM];
+  for (int k = 0; k < NM; ++k) acc[k] = 0.0;
+  for (int q = cv.agg_ptr[I] + c; q < cv.agg_ptr[I + 1]; q += CH) {
+    const int64_t i = cv.agg_nodes[q];
+    double M[ND][NM];
+    node_modes<ND>(m, cv.cen, con, i, I, M);
+    for (int a = 0; a < ND; ++a) {
+      const double rv = (double)r[(i * ND + a) * NRP + rr];
+      for (int k = 0; k < NM; ++k) acc[k] += M[a][k] * rv;
+    }
+  }
+  for (int k = 0; k < NM; ++k) sh[t * NM + k] = acc[k];
+  __syncthreads();
+  if (t < NM * NRP) {
+    const int k = t / NRP, r2 = t % NRP;
+    double v = 0.0;
+    for (int cc = 0; cc < CH; ++cc) v += sh[(cc * NRP + r2) * NM + k];
+    cv.rc[((int64_t)I * NM + k) * NRP + r2] = v;
+  }
+}
+
+// e_c = Ainv r_c, one wave per row (four rows per block), columns split
+// over the lanes with four loads in flight, and the lane sums added by a
+// butterfly.
+template <typename TI, int NRP>
+__global__ void __launch_bounds__(BS) k_coarse_solve(CoarseView cv, const TI *inv,
+                                                     const int *alldone) {
+  if (*alldone) return;
+  const int lane = threadIdx.x % 64;
+  const int row = blockIdx.x * (BS / 64) + threadIdx.x / 64;
+  if (row >= cv.n_c) return;
+  double acc[NRP];
+  for (int r = 0; r < NRP; ++r) acc[r] = 0.0;
+  const TI *ar = inv + (int64_t)row * cv.n_c;
+  int col = lane;
+  for (; col + 192 < cv.n_c; col += 256) {
+    const double a0 = (double)ar[col], a1 = (double)ar[col + 64];
+    const double a2 = (double)ar[col + 128], a3 = (double)ar[col + 192];
+    for (int r = 0; r < NRP; ++r)
+      acc[r] += a0 * cv.rc[(int64_t)col * NRP + r] + a1 * cv.rc[(int64_t)(col + 64) * NRP + r] +
+                a2 * cv.rc[(int64_t)(col + 128) * NRP + r] +
+                a3 * cv.rc[(int64_t)(col + 192) * NRP + r];
+  }
+  for (; col < cv.n_c; col += 64) {
+    const double a = (double)ar[col];
+    for (int r = 0; r < NRP; ++r) acc[r] += a * cv.rc[(int64_t)col * NRP + r];
+  }
+  for (int off = 32; off >= 1; off >>= 1)
+    for (int r = 0; r < NRP; ++r) acc[r] += __shfl_xor(acc[r], off, 64);
+  if (lane < NRP) {
+    double e = 0.0;
+    for (int r = 0; r < NRP; ++r)
+      if (r == lane) e = acc[r];
+    cv.ec[(int64_t)row * NRP + lane] = e;
+  }
+}
+
+// u += P e_c, one thread per (node, rhs). Block 0 also writes r_c . e_c
+// per rhs as one partial set in the (gamma, rr) layout of the PCG's update
+// partials (rr slot 0), summed in row order.
+template <typename T, int ND, int NRP>
+__global__ void __launch_bounds__(256) k_coarse_prolong(tfm_mesh_t m, CoarseView cv,
+                                                        const uint8_t *con, T *u, double *part,
+                                                        const int *alldone) {
+  constexpr int NM = coarse_modes<ND>();
+  __shared__ double sh[256];
+  if (*alldone) return;
+  if (blockIdx.x == 0) {
+    const int rr = threadIdx.x % NRP, c = threadIdx.x / NRP;
+    double d = 0.0;
+    for (int row = c; row < cv.n_c; row += 256 / NRP)
+      d += cv.rc[(int64_t)row * NRP + rr] * cv.ec[(int64_t)row * NRP + rr];
+    sh[threadIdx.x] = d;
+    __syncthreads();
+    if (threadIdx.x < NRP) {
+      double t2 = 0.0;
+      for (int cc = 0; cc < 256 / NRP; ++cc) t2 += sh[cc * NRP + threadIdx.x];
+      part[threadIdx.x] = t2;
+      part[NRP + threadIdx.x] = 0.0;
+    }
+  }
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= m.n_nod * NRP) return;
+  const int64_t i = t / NRP;
+  const int rr = (int)(t % NRP);
+  const int I = cv.agg[i];
+  double M[ND][NM];
+  node_modes<ND>(m, cv.cen, con, i, I, M);
+  double e[NM];
+  for (int k = 0; k < NM; ++k) e[k] = cv.ec[((int64_t)I * NM + k) * NRP + rr];
+  for (int a = 0; a < ND; ++a) {
+    double v = 0.0;
+    for (int k = 0; k < NM; ++k) v += M[a][k] * e[k];
+    T *ua = u + (i * ND + a) * NRP + rr;
+    *ua = (T)((double)*ua + v);
+  }
+}
+
 }  // namespace tfm
 
 // ====================================================================
@@ -1315,7 +1518,32 @@ int launch_spmv(hipStream_t s, const tfm_mesh_t &m, int prec, const void *vals, 
 // ====================================================================
 using namespace tfm;
 
+struct tfm_coarse {
+  tfm_mesh_t m{};  // the mesh (device arrays not owned)
+  int n_agg = 0, nm = 0, n_c = 0;
+  int64_t n_pairs = 0;
+  int32_t *agg = nullptr, *agg_ptr = nullptr, *agg_nodes = nullptr, *slot_agg = nullptr;
+  int32_t *pair_off = nullptr, *pair_node = nullptr, *pair_pos = nullptr;
+  double *cen = nullptr;
+  double *rc = nullptr, *ec = nullptr;  // [n_c * MAXR]
+  int *never = nullptr;                 // a device flag that stays 0 (segment starts)
+  const double *inv64 = nullptr;        // the caller's inverse of P^T K P
+  const float *inv32 = nullptr;
+  CoarseView view() const {
+    return CoarseView{n_agg, n_c, agg, agg_ptr, agg_nodes, slot_agg, pair_off, pair_node,
+                      pair_pos, cen, rc, ec};
+  }
+};
+
+static void coarse_free(tfm_coarse *h) {
+  for (void *ptr : {(void *)h->agg, (void *)h->agg_ptr, (void *)h->agg_nodes, (void *)h->slot_agg,
+                    (void *)h->pair_off, (void *)h->pair_node, (void *)h->pair_pos,
+                    (void *)h->cen, (void *)h->rc, (void *)h->ec, (void *)h->never})
+    if (ptr) (void)hipFree(ptr);
+}
+
 struct tfm_pcg {
+  tfm_coarse *coarse = nullptr;  // attached coarse space (TFM_PC_TWO_LEVEL)
   int64_t n = 0;  // vector length (DOFs)
   int nrhs = 0;   // capacity
   int nrp = 1;    // capacity padded to a power of two
@@ -1371,7 +1599,7 @@ static int pcg_alloc(tfm_pcg *h) {
   // (the 64-lane team), n / 4 covers it for one DOF per node
   h->pq_cap = (int)std::max<int64_t>(RBA, n / 4 + 1);
   TFM_CHECK_HIP(hipMalloc(&h->part_pq, sizeof(double) * (size_t)h->pq_cap * nrp));
-  TFM_CHECK_HIP(hipMalloc(&h->part_upd, sizeof(double) * RBW * 2 * nrp));
+  TFM_CHECK_HIP(hipMalloc(&h->part_upd, sizeof(double) * (RBW + CGB) * 2 * nrp));
   TFM_CHECK_HIP(hipMalloc(&h->part_aux, sizeof(double) * RBW * nrp));
   h->res_cap = (int)std::max<int64_t>(RBV, n / 8 + 1);  // >= n_nod / 8 node teams
   TFM_CHECK_HIP(hipMalloc(&h->part_res, sizeof(double) * (size_t)h->res_cap * 2));
@@ -1443,6 +1671,100 @@ extern "C" int tfm_pcg_timing(tfm_pcg_t *h, int32_t enable, double *spmv_ms,
   return TFM_OK;
 }
 
+extern "C" int tfm_coarse_create(tfm_coarse_t **out, const tfm_mesh_t *m, int32_t n_agg,
+                                 const int32_t *agg, const int32_t *agg_ptr,
+                                 const int32_t *agg_nodes, const double *centroids,
+                                 const int32_t *slot_agg, const int32_t *pair_off,
+                                 int64_t n_pairs, const int32_t *pair_node,
+                                 const int32_t *pair_pos) {
+  TFM_REQUIRE(out && m && agg && agg_ptr && agg_nodes && centroids && slot_agg && pair_off,
+              "null argument");
+  TFM_REQUIRE(n_pairs == 0 || (pair_node && pair_pos), "null pair arrays");
+  TFM_REQUIRE(n_agg >= 1 && n_pairs >= 0, "bad aggregate or pair count");
+  TFM_REQUIRE(m->ndof == 1 || m->ndof == m->dim, "ndof must be 1 or the dimension");
+  const int nm = m->ndof == 3 ? 6 : m->ndof == 2 ? 3 : 1;
+  TFM_REQUIRE((int64_t)n_agg * nm <= 32768, "coarse space above 32768 unknowns");
+  auto *h = new tfm_coarse();
+  h->m = *m;
+  h->n_agg = n_agg;
+  h->nm = nm;
+  h->n_c = n_agg * nm;
+  h->n_pairs = n_pairs;
+  auto copy = [&](auto **dst, const auto *src, size_t count) -> int {
+    using E = std::remove_pointer_t<std::remove_reference_t<decltype(*dst)>>;
+    TFM_CHECK_HIP(hipMalloc((void **)dst, sizeof(E) * std::max<size_t>(count, 1)));
+    if (count) TFM_CHECK_HIP(hipMemcpy(*dst, src, sizeof(E) * count, hipMemcpyDeviceToDevice));
+    return TFM_OK;
+  };
+  int rc = TFM_OK;
+  if (rc == TFM_OK) rc = copy(&h->agg, agg, (size_t)m->n_nod);
+  if (rc == TFM_OK) rc = copy(&h->agg_ptr, agg_ptr, (size_t)n_agg + 1);
+  if (rc == TFM_OK) rc = copy(&h->agg_nodes, agg_nodes, (size_t)m->n_nod);
+  if (rc == TFM_OK) rc = copy(&h->cen, centroids, (size_t)n_agg * m->dim);
+  if (rc == TFM_OK) rc = copy(&h->slot_agg, slot_agg, (size_t)n_agg * 27);
+  if (rc == TFM_OK) rc = copy(&h->pair_off, pair_off, (size_t)n_agg * 27 + 1);
+  if (rc == TFM_OK) rc = copy(&h->pair_node, pair_node, (size_t)n_pairs);
+  if (rc == TFM_OK) rc = copy(&h->pair_pos, pair_pos, (size_t)n_pairs);
+  if (rc == TFM_OK && hipMalloc(&h->rc, sizeof(double) * h->n_c * MAXR) != hipSuccess) rc = TFM_ERR_HIP;
+  if (rc == TFM_OK && hipMalloc(&h->ec, sizeof(double) * h->n_c * MAXR) != hipSuccess) rc = TFM_ERR_HIP;
+  if (rc == TFM_OK && hipMalloc(&h->never, sizeof(int)) != hipSuccess) rc = TFM_ERR_HIP;
+  if (rc == TFM_OK && hipMemset(h->never, 0, sizeof(int)) != hipSuccess) rc = TFM_ERR_HIP;
+  if (rc != TFM_OK) {
+    set_error("allocation of the coarse space failed");
+    coarse_free(h);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return TFM_OK;
+}
+
+extern "C" int tfm_coarse_destroy(tfm_coarse_t *h) {
+  if (!h) return TFM_OK;
+  coarse_free(h);
+  delete h;
+  return TFM_OK;
+}
+
+extern "C" int tfm_coarse_size(const tfm_coarse_t *h, int32_t *n_c) {
+  TFM_REQUIRE(h && n_c, "null argument");
+  *n_c = h->n_c;
+  return TFM_OK;
+}
+
+extern "C" int tfm_coarse_operator(tfm_stream_t stream, const tfm_coarse_t *h,
+                                   const tfm_system_t *sys, double *Ac) {
+  TFM_REQUIRE(h && sys && sys->mesh && sys->val64 && sys->con && Ac, "null argument");
+  const tfm_mesh_t &m = *sys->mesh;
+  TFM_REQUIRE(m.n_nod == h->m.n_nod && m.ndof == h->m.ndof, "system and coarse space differ");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t nn = (int64_t)h->n_c * h->n_c;
+  hipLaunchKernelGGL((k_fill_zero<double>), dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s,
+                     nn, Ac);
+  TFM_CHECK_LAUNCH();
+  const CoarseView cv = h->view();
+  switch (m.ndof) {
+    case 1: hipLaunchKernelGGL(k_coarse_operator<1>, dim3((unsigned)h->n_agg * 27), dim3(64), 0, s, m, cv, sys->val64, sys->con, Ac); break;
+    case 2: hipLaunchKernelGGL(k_coarse_operator<2>, dim3((unsigned)h->n_agg * 27), dim3(64), 0, s, m, cv, sys->val64, sys->con, Ac); break;
+    default: hipLaunchKernelGGL(k_coarse_operator<3>, dim3((unsigned)h->n_agg * 27), dim3(64), 0, s, m, cv, sys->val64, sys->con, Ac); break;
+  }
+  TFM_CHECK_LAUNCH();
+  return TFM_OK;
+}
+
+extern "C" int tfm_coarse_set_inverse(tfm_coarse_t *h, const double *inv64, const float *inv32) {
+  TFM_REQUIRE(h && inv64, "null argument");
+  h->inv64 = inv64;
+  h->inv32 = inv32;
+  return TFM_OK;
+}
+
+extern "C" int tfm_pcg_set_coarse(tfm_pcg_t *p, tfm_coarse_t *c) {
+  TFM_REQUIRE(p != nullptr, "null solver handle");
+  p->coarse = c;
+  return TFM_OK;
+}
+
 extern "C" int tfm_pcg_fp64_fallback(const tfm_pcg_t *h, int32_t *fp64_iters) {
   TFM_REQUIRE(h != nullptr && fp64_iters != nullptr, "null argument");
   *fp64_iters = h->fallback_iters;
@@ -1462,7 +1784,35 @@ template <typename T> struct Ops {
       spmv_dir;
   std::function<int(const T *, T *)> spmv;
   std::function<int(const double *, const double *, double *, double *, double *)> residual;
+  // two-level preconditioner: u += P Ac^{-1} P^T r and the partial sets of
+  // (P^T r).(Ac^{-1} P^T r) at `part`; skipped once *flag is set (empty if unused)
+  std::function<int(const T *, T *, double *, const int *)> coarse;
 };
+
+// The coarse correction's three launches: r_c = P^T r, e_c = Ac^{-1} r_c
+// (with the partials of r_c . e_c), u += P e_c.
+template <typename T, int ND, int NRP>
+std::function<int(const T *, T *, double *, const int *)> coarse_ops(hipStream_t s,
+                                                                     const tfm_mesh_t &m,
+                                                                     const tfm_coarse &c,
+                                                                     const uint8_t *con,
+                                                                     const T *inv) {
+  const CoarseView cv = c.view();
+  const tfm_mesh_t mm = m;
+  return [=](const T *r, T *u, double *part, const int *flag) -> int {
+    hipLaunchKernelGGL((k_coarse_restrict<T, ND, NRP>), dim3((unsigned)cv.n_agg), dim3(BS), 0, s,
+                       mm, cv, con, r, flag);
+    TFM_CHECK_LAUNCH();
+    hipLaunchKernelGGL((k_coarse_solve<T, NRP>), dim3((unsigned)((cv.n_c + 3) / 4)), dim3(BS), 0,
+                       s, cv, inv, flag);
+    TFM_CHECK_LAUNCH();
+    hipLaunchKernelGGL((k_coarse_prolong<T, ND, NRP>),
+                       dim3((unsigned)((mm.n_nod * NRP + 255) / 256)), dim3(256), 0, s, mm, cv, con,
+                       u, part, flag);
+    TFM_CHECK_LAUNCH();
+    return TFM_OK;
+  };
+}
 
 int sum_residual(tfm_pcg *h, hipStream_t s, unsigned nblk, double *rr, double *bb) {
   hipLaunchKernelGGL(k_sum2, dim3(1), dim3(1024), 0, s, (const double *)h->part_res, (int)nblk, h->res2);
@@ -1497,6 +1847,10 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
   // whole multiples of the CU count, so that no CU runs a second round alone
   const unsigned gv = std::min<unsigned>(ops.scal ? std::max(RBV, RBW / NRP) : vcap,
                                          round_to_cus(grid_for(n_nod * NRP, BSV, RBW)));
+  // the coarse solve's partial sets follow the update's in part_upd
+  const int n_part = (int)gv + (ops.coarse ? 1 : 0);
+  double *const part_coarse = h->part_upd + (size_t)gv * 2 * NRP;
+  const int *never = h->coarse ? h->coarse->never : nullptr;
   std::vector<double> htol(NRP, -1.0), bnorm(NRP, 0.0), rprev(NRP, 0.0), rnow(NRP, 0.0);
   std::vector<int> fin(NRP, 1), stall(NRP, 0);
   for (int r = 0; r < nrhs; ++r) iters[r] = 0;
@@ -1542,8 +1896,12 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
     hipLaunchKernelGGL((k_pcg_seg_dots<T, ND, NRP>), dim3(gv), dim3(BS), 0, s, n_nod,
                        (const T *)rv, uv, (const T *)pv, minv, st);
     TFM_CHECK_LAUNCH();
+    if (ops.coarse) {
+      const int rc = ops.coarse(rv, uv, part_coarse, never);
+      if (rc != TFM_OK) return rc;
+    }
     hipLaunchKernelGGL((k_pcg_seg_apply<T, ND, NRP>), dim3(gv), dim3(BS), 0, s, n_nod, it, mask,
-                       cont, running, seg, (const T *)uv, pv, sv, st, (int)gv);
+                       cont, running, seg, (const T *)uv, pv, sv, st, n_part, (int)gv);
     TFM_CHECK_LAUNCH();
     if (cont) return ops.spmv(pv, sv);
     return TFM_OK;
@@ -1584,13 +1942,13 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
     for (; it < stop; ++it) {
       if (ops.scal) {
         hipLaunchKernelGGL((k_pcg_scal_dir<NRP>), dim3(1), dim3(BS), 0, s, it,
-                           (const double *)h->part_upd, (int)gv, st);
+                           (const double *)h->part_upd, n_part, st);
         TFM_CHECK_LAUNCH();
       }
       const bool timed = h->timing && it % h->timing == 0;
       if (timed) TFM_CHECK_HIP(hipEventRecord(h->ev[2 * (it - start)], s));
       unsigned npq = 0;
-      int rc = ops.spmv_dir(uv, pv, sv, it, h->part_upd, (int)gv, st, &npq);
+      int rc = ops.spmv_dir(uv, pv, sv, it, h->part_upd, n_part, st, &npq);
       if (rc != TFM_OK) return rc;
       if (timed) TFM_CHECK_HIP(hipEventRecord(h->ev[2 * (it - start) + 1], s));
       if (ops.scal) {
@@ -1606,6 +1964,10 @@ int run_pcg(tfm_pcg *h, hipStream_t s, int64_t n_nod, Ops<T> &ops, const double 
                            (const double *)h->part_pq, (int)npq);
       }
       TFM_CHECK_LAUNCH();
+      if (ops.coarse) {
+        rc = ops.coarse(rv, uv, part_coarse, h->alldone);
+        if (rc != TFM_OK) return rc;
+      }
     }
     TFM_CHECK_HIP(hipMemcpyAsync(h->h_done, h->done, sizeof(int) * NRP, hipMemcpyDeviceToHost, s));
     TFM_CHECK_HIP(hipMemcpyAsync(h->h_scal, h->scal + (size_t)start * NRP * NSCAL,
@@ -1782,7 +2144,17 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
   const int64_t n = m.n_nod * m.ndof;
   TFM_REQUIRE(n == h->n, "system size differs from the solver handle");
   TFM_REQUIRE(nrhs >= 1 && nrhs <= h->nrhs, "nrhs exceeds the solver handle");
-  TFM_REQUIRE(pc == TFM_PC_JACOBI || pc == TFM_PC_BLOCK_JACOBI, "unknown preconditioner");
+  TFM_REQUIRE(pc == TFM_PC_JACOBI || pc == TFM_PC_BLOCK_JACOBI || pc == TFM_PC_TWO_LEVEL,
+              "unknown preconditioner");
+  const tfm_coarse *cs = pc == TFM_PC_TWO_LEVEL ? h->coarse : nullptr;
+  if (pc == TFM_PC_TWO_LEVEL) {
+    TFM_REQUIRE(cs != nullptr, "TFM_PC_TWO_LEVEL needs a coarse space (tfm_pcg_set_coarse)");
+    TFM_REQUIRE(cs->m.n_nod == m.n_nod && cs->m.ndof == m.ndof,
+                "the coarse space belongs to another mesh");
+    TFM_REQUIRE(cs->inv64 != nullptr && (prec == TFM_PREC_FP64 || cs->inv32 != nullptr),
+                "the coarse inverse is missing (tfm_coarse_set_inverse)");
+    pc = TFM_PC_BLOCK_JACOBI;
+  }
   TFM_REQUIRE(prec == TFM_PREC_FP64 || prec == TFM_PREC_MIXED, "unknown precision mode");
   TFM_REQUIRE(sys->con && sys->diag_blocks, "system needs con and diag_blocks");
   TFM_REQUIRE(prec == TFM_PREC_FP64 ? sys->val64 != nullptr : sys->val32 != nullptr,
@@ -1818,6 +2190,7 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
         return spmv_launch<double, ND, NRP>(s, m, sys->val64, xi, yo);
       };
       ops.residual = residual;
+      if (cs) ops.coarse = coarse_ops<double, ND, NRP>(s, m, *cs, sys->con, cs->inv64);
       return run_pcg<double, ND, NRP>(h, s, m.n_nod, ops, b, x, nrhs, rtol, inner_rtol,
                                       final_rtol, its, relres, out);
     };
@@ -1837,6 +2210,7 @@ extern "C" int tfm_pcg_solve(tfm_pcg_t *h, tfm_stream_t stream, const tfm_system
         return spmv_launch<float, ND, NRP>(s, m, sys->val32, xi, yo);
       };
       ops.residual = residual;
+      if (cs) ops.coarse = coarse_ops<float, ND, NRP>(s, m, *cs, sys->con, cs->inv32);
       rc = run_pcg<float, ND, NRP>(h, s, m.n_nod, ops, b, x, nrhs, rtol, inner_rtol, final_rtol,
                                    iters, relres, outer);
       // FP32 rounding of a matrix whose element stiffnesses span many orders

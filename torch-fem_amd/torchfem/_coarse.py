@@ -1,0 +1,152 @@
+"""Coarse space of the two-level preconditioner (TFM_PC_TWO_LEVEL), in the
+place of the aggregation AMG the reference reaches through AmgX
+(sparse.py:422-442, amgx.py:71-98).
+
+Nodes are grouped into aggregates by a grid of cubic cells over the mesh's
+bounding box. The cell is at least 1.01 times the largest element extent, so
+K couples an aggregate only to the 27 cells around it. The cells are sized
+for about one coarse unknown per 128 fine ones and at most COARSE_SIZE
+(256 aggregates with 6 rigid-body modes each in 3-D, 512 with 3 in 2-D,
+1,536 with the constant for a scalar field). The index work below runs once
+per pattern with torch on the device. The coarse operator P^T K P is formed
+per system by a HIP kernel (tfm_coarse_operator), and its dense inverse by
+torch.linalg (Cholesky) on the device.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import weakref
+
+import torch
+from torch import Tensor
+
+from . import _lib as L
+
+COARSE_SIZE = 1536  # most coarse unknowns (timed on the 64^3 cube: 768, 1536, 3072)
+
+
+class CoarseSpace:
+    """Aggregates of one node-block pattern and their C-ABI handle."""
+
+    def __init__(self, pat, size: int | None = None):
+        X = pat.nodes
+        dev = X.device
+        D = pat.dim
+        nd = pat.ndof
+        self.nm = {3: 6, 2: 3, 1: 1}[nd]
+        if size is None:
+            # about one coarse unknown per 128 fine ones, at most COARSE_SIZE
+            size = min(COARSE_SIZE, max(self.nm, pat.n_dofs // 128))
+        target = max(1, size // self.nm)
+        lo = X.min(0).values
+        ext = X.max(0).values - lo
+        if pat.n_elem:
+            Xe = X[pat.elements.long()]
+            emax = float((Xe.max(1).values - Xe.min(1).values).max())
+        else:
+            emax = float(ext.max()) if X.shape[0] else 1.0
+        emax = max(emax, 1e-300)
+        ext_eff = torch.clamp(ext, min=emax)
+        vol = float(torch.prod(ext_eff))
+        cell = max((vol / target) ** (1.0 / D), 1.01 * emax)
+        nc = [max(1, int(math.ceil(float(e) / cell))) for e in ext_eff]
+        nc_t = torch.tensor(nc, device=dev)
+        cidx = torch.minimum(torch.floor((X - lo) / cell).long().clamp(min=0), nc_t - 1)
+        if D == 2:
+            cidx = torch.cat([cidx, torch.zeros_like(cidx[:, :1])], 1)
+            nc = nc + [1]
+        lin = (cidx[:, 0] * nc[1] + cidx[:, 1]) * nc[2] + cidx[:, 2]
+        uniq, agg = torch.unique(lin, return_inverse=True)
+        n_agg = int(uniq.numel())
+        self.n_agg = n_agg
+        # nodes of each aggregate, ascending
+        agg_nodes = torch.sort(agg, stable=True).indices
+        counts = torch.bincount(agg, minlength=n_agg)
+        agg_ptr = torch.zeros(n_agg + 1, dtype=torch.int64, device=dev)
+        agg_ptr[1:] = torch.cumsum(counts, 0)
+        cen = torch.zeros(n_agg, D, dtype=torch.float64, device=dev).index_add_(0, agg, X)
+        cen = cen / counts[:, None].double()
+        acell = torch.stack([uniq // (nc[1] * nc[2]), (uniq // nc[2]) % nc[1], uniq % nc[2]], 1)
+        # cell -> aggregate, then the aggregate of each of the 27 slots
+        cmap = torch.full((nc[0] * nc[1] * nc[2],), -1, dtype=torch.int64, device=dev)
+        cmap[uniq] = torch.arange(n_agg, device=dev)
+        off = torch.stack(torch.meshgrid(*[torch.arange(-1, 2, device=dev)] * 3, indexing="ij"),
+                          -1).reshape(27, 3)
+        nb = acell[:, None, :] + off[None, :, :]
+        ok = ((nb >= 0) & (nb < torch.tensor(nc, device=dev))).all(-1)
+        nbl = (nb[..., 0] * nc[1] + nb[..., 1]) * nc[2] + nb[..., 2]
+        slot_agg = torch.where(ok, cmap[nbl.clamp(min=0, max=cmap.numel() - 1)],
+                               torch.full_like(nbl, -1))
+        # node blocks (i, p) grouped by (aggregate of i, slot of the aggregate of j)
+        deg = (pat.adj_ptr[1:] - pat.adj_ptr[:-1]).long()
+        rows = torch.repeat_interleave(torch.arange(pat.n_nod, device=dev), deg)
+        pos = torch.arange(rows.numel(), device=dev) - pat.adj_ptr[:-1].long()[rows]
+        cols = pat.adj.long()
+        d = acell[agg[cols]] - acell[agg[rows]] + 1
+        if rows.numel() and (int(d.min()) < 0 or int(d.max()) > 2):
+            raise RuntimeError("an element spans more than two aggregate cells")
+        slot = (d[:, 0] * 3 + d[:, 1]) * 3 + d[:, 2]
+        key = agg[rows] * 27 + slot
+        order = torch.sort(key, stable=True).indices
+        pair_node = rows[order].to(torch.int32).contiguous()
+        pair_pos = pos[order].to(torch.int32).contiguous()
+        kc = torch.bincount(key, minlength=n_agg * 27)
+        pair_off = torch.zeros(n_agg * 27 + 1, dtype=torch.int64, device=dev)
+        pair_off[1:] = torch.cumsum(kc, 0)
+        arrays = dict(
+            agg=agg.to(torch.int32).contiguous(),
+            agg_ptr=agg_ptr.to(torch.int32).contiguous(),
+            agg_nodes=agg_nodes.to(torch.int32).contiguous(),
+            cen=cen.contiguous(),
+            slot_agg=slot_agg.to(torch.int32).contiguous(),
+            pair_off=pair_off.to(torch.int32).contiguous(),
+        )
+        self.agg = arrays["agg"]
+        h = L.P()
+        with torch.cuda.device(dev):
+            L.check(L.lib().tfm_coarse_create(
+                C.byref(h), C.byref(pat.mesh), n_agg, L.ptr(arrays["agg"]),
+                L.ptr(arrays["agg_ptr"]), L.ptr(arrays["agg_nodes"]), L.ptr(arrays["cen"]),
+                L.ptr(arrays["slot_agg"]), L.ptr(arrays["pair_off"]), int(pair_node.numel()),
+                L.ptr(pair_node), L.ptr(pair_pos)))
+        self.h = h
+        self._release = weakref.finalize(self, L.lib().tfm_coarse_destroy, C.c_void_p(h.value))
+        nc_out = L.I32()
+        L.check(L.lib().tfm_coarse_size(h, C.byref(nc_out)))
+        self.n_c = int(nc_out.value)
+        self.pat = pat
+
+    def inverse(self, system) -> tuple[Tensor, Tensor]:
+        """(P^T K P)^{-1} of an assembled system, FP64 and FP32, symmetric
+        positive definite. The operator is scaled to a unit diagonal first
+        (aggregates of very different stiffness, as in density designs with
+        void elements, and the translation / rotation scales then meet at one
+        scale); coarse modes that vanish (aggregates whose DOFs are all
+        constrained) get a unit diagonal, and a 1e-12 shift keeps the factor
+        defined when an aggregate's modes are dependent (their prolongation is
+        zero either way). If the Cholesky factorization still fails, the
+        eigenvalues below 1e-12 of the largest are dropped."""
+        pat = self.pat
+        Ac = torch.empty(self.n_c, self.n_c, dtype=torch.float64, device=pat.device)
+        L.check(L.lib().tfm_coarse_operator(pat.stream(), self.h, C.byref(system.sys), L.ptr(Ac)))
+        Ac = 0.5 * (Ac + Ac.T)
+        dg = Ac.diagonal().clone()
+        top = float(dg.abs().max()) if self.n_c else 1.0
+        top = top if top > 0.0 else 1.0
+        dead = dg <= 1e-14 * top
+        dg = torch.where(dead, torch.ones_like(dg), dg)
+        sc = 1.0 / torch.sqrt(dg)
+        A = sc[:, None] * Ac * sc[None, :]
+        A.diagonal().copy_(torch.where(dead, torch.ones_like(dg), A.diagonal() + 1e-12))
+        Lc, info = torch.linalg.cholesky_ex(A)
+        if int(info) != 0:
+            w, V = torch.linalg.eigh(A)
+            w = torch.where(w > 1e-12 * float(w.abs().max()), 1.0 / w, torch.zeros_like(w))
+            inv = (V * w) @ V.T
+        else:
+            inv = torch.cholesky_inverse(Lc)
+        inv = sc[:, None] * (0.5 * (inv + inv.T)) * sc[None, :]
+        inv = inv.contiguous()
+        return inv, inv.float().contiguous()

@@ -28,7 +28,7 @@ TFM_ERR_JACOBIAN = 4
 ETYPE = {"Quad1": 1, "Tetra1": 2, "Hexa1": 3, "Hexa2": 4}
 MAT_ISO, MAT_GENERAL, MAT_HEAT_ISO, MAT_HEAT_GENERAL = 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
-PC_JACOBI, PC_BLOCK_JACOBI = 0, 1
+PC_JACOBI, PC_BLOCK_JACOBI, PC_TWO_LEVEL = 0, 1, 2
 FP32_PLAIN, FP32_DIFF = 0, 1
 
 P = C.c_void_p
@@ -149,6 +149,12 @@ _SIGNATURES: dict[str, tuple[list[Any], Any]] = {
     ),
     "tfm_pcg_timing": ([P, I32, C.POINTER(F64), C.POINTER(I64)], C.c_int),
     "tfm_pcg_fp64_fallback": ([P, C.POINTER(I32)], C.c_int),
+    "tfm_coarse_create": ([C.POINTER(P), P, I32, P, P, P, P, P, P, I64, P, P], C.c_int),
+    "tfm_coarse_destroy": ([P], C.c_int),
+    "tfm_coarse_size": ([P, C.POINTER(I32)], C.c_int),
+    "tfm_coarse_operator": ([P, P, P, P], C.c_int),
+    "tfm_coarse_set_inverse": ([P, P, P], C.c_int),
+    "tfm_pcg_set_coarse": ([P, P], C.c_int),
     "tfm_csr_pcg_solve": (
         [P, P, I64, P, P, P, P, P, F64, C.POINTER(I32), C.POINTER(F64)],
         C.c_int,

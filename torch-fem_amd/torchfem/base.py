@@ -37,11 +37,12 @@ from . import _lib as L
 from ._pattern import DevicePattern, SolverView, locality_order
 from .elements import Element
 from .materials import Material
+from ._coarse import CoarseSpace
 from ._finite import Kinematics, solve_finite
 from .sparse import CachedSolve, PCGSolver, default_precision
 
 _PREC = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED}
-_PC = {"jacobi": L.PC_JACOBI, "block_jacobi": L.PC_BLOCK_JACOBI}
+_PC = {"jacobi": L.PC_JACOBI, "block_jacobi": L.PC_BLOCK_JACOBI, "two_level": L.PC_TWO_LEVEL}
 
 
 def _n_out(kind: int, dim: int) -> int:
@@ -452,14 +453,30 @@ class FEM(ABC):
         if perm is not None:  # to the solver's node order
             b, x = b[:, perm].contiguous(), x[:, perm].contiguous()
         solver = self._solver()
+        coarse = self._coarse_for(system) if cfg["pc"] == "two_level" else None
         # once the FP32 iterations of a mixed solve broke down on this system
         # (its FP32 operator is not positive definite), later solves with the
         # same matrix (the adjoint, further Newton steps) run in FP64 directly
         precision = "fp64" if getattr(system, "fp32_broke", False) else cfg["precision"]
-        info = solver.solve_system(system.sys, b, x, precision=precision,
-                                   preconditioner=cfg["pc"], rtol=cfg["stol"],
-                                   inner_rtol=cfg["inner_rtol"],
-                                   final_rtol=(0.7 if adjoint else 0.25) * cfg["stol"])
+        kw = dict(precision=precision, rtol=cfg["stol"], inner_rtol=cfg["inner_rtol"],
+                  final_rtol=(0.7 if adjoint else 0.25) * cfg["stol"])
+        use_two = coarse is not None and not getattr(system, "coarse_failed", False)
+        if use_two:
+            try:
+                info = solver.solve_system(system.sys, b, x, preconditioner="two_level",
+                                           coarse=coarse, **kw)
+            except RuntimeError:
+                # the coarse correction did not reach the tolerance (seen on
+                # density designs with a 1e9 stiffness contrast): block-Jacobi
+                # continues from the iterate, here and for later solves with
+                # this matrix
+                system.coarse_failed = True
+                use_two = False
+        if not use_two:
+            pc = "block_jacobi" if cfg["pc"] == "two_level" else cfg["pc"]
+            info = solver.solve_system(system.sys, b, x, preconditioner=pc, **kw)
+            if cfg["pc"] == "two_level":
+                info["two_level_fallback"] = True
         if info.get("fp64_fallback", 0) > 0:
             system.fp32_broke = True
         self._last_pcg = info
@@ -468,6 +485,23 @@ class FEM(ABC):
             out[:, perm] = x
             return out
         return x
+
+    def _coarse_for(self, system):
+        """The two-level preconditioner's coarse space for the system's
+        pattern (built once per pattern), holding the inverse of the system's
+        coarse operator (formed once per system); returns its handle."""
+        pat = getattr(system, "_pat", None) or self._pattern
+        cache = self.__dict__.setdefault("_coarse_spaces", {})
+        cs = cache.get(id(pat))
+        if cs is None or cs.pat is not pat:
+            cs = CoarseSpace(pat)
+            cache[id(pat)] = cs
+        inv = getattr(system, "_coarse_inv", None)
+        if inv is None:
+            inv = cs.inverse(system)
+            system._coarse_inv = inv
+        L.check(L.lib().tfm_coarse_set_inverse(cs.h, L.ptr(inv[0]), L.ptr(inv[1])))
+        return cs.h
 
     def _solver_view(self):
         """The node order the PCG runs in. A mesh whose numbering puts
@@ -785,8 +819,11 @@ class FEM(ABC):
 
         Extra keywords of this build: `precision` ("mixed": FP32-inner PCG with
         FP64 refinement, the default; "fp64": all-FP64 PCG), `preconditioner`
-        ("block_jacobi" or "jacobi") and `inner_rtol` (mixed precision: the factor by
-        which the FP32 recursive residual drops between two reliable updates).
+        ("block_jacobi", "jacobi", or "two_level": block-Jacobi with an additive
+        coarse correction on node aggregates and their rigid-body modes, in the
+        place of the reference's AMG) and `inner_rtol` (mixed precision: the
+        factor by which the FP32 recursive residual drops between two reliable
+        updates).
         """
         finite = getattr(self.material, "incremental", False)
         if nlgeom and not finite:

@@ -139,17 +139,20 @@ class PCGSolver:
 
     def solve_system(self, sys: L.System, b: Tensor, x: Tensor, precision="mixed",
                      preconditioner="block_jacobi", rtol=1e-10, inner_rtol=0.1,
-                     final_rtol=None) -> dict:
+                     final_rtol=None, coarse=None) -> dict:
         """K_bc x = b for the rows of b [nrhs, n] (x in/out), on an assembled system.
         Mixed precision: the last FP32 segment aims at final_rtol (default
         0.7 rtol) so that the FP64 residual ends below rtol with a margin."""
         nrhs = b.shape[0]
         self._ensure(nrhs)
+        # the coarse space of preconditioner="two_level" (a tfm_coarse handle)
+        L.check(L.lib().tfm_pcg_set_coarse(self._h, coarse))
         its = (L.I32 * nrhs)()
         rel = (L.F64 * nrhs)()
         outer = L.I32()
         prec = L.PREC_MIXED if precision == "mixed" else L.PREC_FP64
-        pc = L.PC_BLOCK_JACOBI if preconditioner == "block_jacobi" else L.PC_JACOBI
+        pc = {"block_jacobi": L.PC_BLOCK_JACOBI, "two_level": L.PC_TWO_LEVEL}.get(
+            preconditioner, L.PC_JACOBI)
         t0 = time.perf_counter()
         rc = L.lib().tfm_pcg_solve(self._h, L.stream_of(self.device), C.byref(sys), prec, pc,
                                    L.ptr(b), L.ptr(x), nrhs, float(rtol), float(inner_rtol),

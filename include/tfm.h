@@ -379,6 +379,16 @@ int tfm_coarse_set_inverse(tfm_coarse_t *h, const double *inv64, const float *in
  * detaches). */
 int tfm_pcg_set_coarse(tfm_pcg_t *p, tfm_coarse_t *c);
 
+/* Inverse of a dense symmetric positive definite matrix, in place: a
+ * [n*n] row-major on the device, 1 <= n <= 32768. work holds
+ * tfm_spd_inverse_work(n) doubles; status (device int32, zeroed by the
+ * caller) becomes k + 1 if the k-th pivot is not positive, and a then holds
+ * no inverse. Block Gauss-Jordan with 64-column pivot blocks, three kernels
+ * per block. No counterpart in the reference (AmgX factors its coarsest
+ * level internally): the coarse inverse of TFM_PC_TWO_LEVEL. */
+int tfm_spd_inverse_work(int32_t n, int64_t *count);
+int tfm_spd_inverse(tfm_stream_t s, int32_t n, double *a, double *work, int32_t *status);
+
 /* Preconditioned CG for a general SPD CSR matrix (FP64, Jacobi). Like
  * tfm_pcg_solve it fails with TFM_ERR_NOCONV as soon as an iteration finds
  * p.Ap <= 0 (the matrix is not positive definite). */

@@ -163,6 +163,29 @@ def test_load_cases_and_constrained_aggregates(precision):
         assert rel(U[c], u) < 1e-9
 
 
+@pytest.mark.parametrize("n", [1, 7, 64, 100, 257])
+def test_spd_inverse(n):
+    """tfm_spd_inverse (block Gauss-Jordan, 64-column pivot blocks) against
+    torch.linalg.inv on SPD matrices with condition number 1e6, sizes below,
+    at and across block boundaries; an indefinite matrix reports the first
+    pivot that is not positive."""
+    from torchfem._coarse import spd_inverse
+    g = torch.Generator().manual_seed(n)
+    Q, _ = torch.linalg.qr(torch.randn(n, n, generator=g, dtype=DT))
+    A = ((Q * torch.logspace(-6, 0, n, dtype=DT)) @ Q.T).cuda()
+    A = 0.5 * (A + A.T)
+    X = spd_inverse(A.clone())
+    ref = torch.linalg.inv(A)
+    assert X is not None
+    # the inverse's error is bounded by cond(A) * eps relative to its norm
+    assert rel(X, ref) < 1e-8
+    assert float((X @ A - torch.eye(n, dtype=DT, device="cuda")).abs().max()) < 1e-8
+    if n > 1:
+        B = torch.eye(n, dtype=DT, device="cuda")
+        B[n - 1, n - 1] = -1.0
+        assert spd_inverse(B) is None
+
+
 def test_iterations_drop():
     """24^3 Hexa1 compression (the bench's load on a smaller cube): the
     coarse correction at least halves the PCG iterations of block-Jacobi,

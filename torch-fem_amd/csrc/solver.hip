@@ -1402,15 +1402,32 @@ __global__ void __launch_bounds__(64) k_coarse_operator(tfm_mesh_t m, CoarseView
   }
 }
 
+// Sums of v over the lanes of each wave that share lane % NRP (a butterfly
+// in a fixed order), then over the block's waves in wave order, for the
+// values of one block of BS threads; the result is valid in threads
+// 0 .. NRP - 1 (rhs = thread). Every thread of the block must call it.
+template <int NRP>
+__device__ __forceinline__ double block_sum_by_rhs(double v, double *sh) {
+  for (int off = 32; off >= NRP; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  if (lane < NRP) sh[wave * NRP + lane] = v;
+  __syncthreads();
+  double tsum = 0.0;
+  if (threadIdx.x < NRP)
+    for (int w = 0; w < BS / 64; ++w) tsum += sh[w * NRP + threadIdx.x];
+  __syncthreads();
+  return tsum;
+}
+
 // r_c = P^T r: one block per aggregate; thread t takes rhs t % NRP of every
-// (BS / NRP)-th node of the aggregate, and the block adds the thread sums in
-// thread order.
+// (BS / NRP)-th node of the aggregate, and the block adds the thread sums by
+// waves (block_sum_by_rhs).
 template <typename T, int ND, int NRP>
 __global__ void __launch_bounds__(BS) k_coarse_restrict(tfm_mesh_t m, CoarseView cv,
                                                         const uint8_t *con, const T *r,
                                                         const int *alldone) {
   constexpr int NM = coarse_modes<ND>(), CH = BS / NRP;
-  __shared__ double sh[BS * NM];
+  __shared__ double sh[BS / 64 * NRP];
   if (*alldone) return;
   const int I = blockIdx.x, t = threadIdx.x, rr = t % NRP, c = t / NRP;
   double acc[NM];
@@ -1424,13 +1441,9 @@ __global__ void __launch_bounds__(BS) k_coarse_restrict(tfm_mesh_t m, CoarseView
       for (int k = 0; k < NM; ++k) acc[k] += M[a][k] * rv;
     }
   }
-  for (int k = 0; k < NM; ++k) sh[t * NM + k] = acc[k];
-  __syncthreads();
-  if (t < NM * NRP) {
-    const int k = t / NRP, r2 = t % NRP;
-    double v = 0.0;
-    for (int cc = 0; cc < CH; ++cc) v += sh[(cc * NRP + r2) * NM + k];
-    cv.rc[((int64_t)I * NM + k) * NRP + r2] = v;
+  for (int k = 0; k < NM; ++k) {
+    const double v = block_sum_by_rhs<NRP>(acc[k], sh);
+    if (t < NRP) cv.rc[((int64_t)I * NM + k) * NRP + t] = v;
   }
 }
 
@@ -1472,24 +1485,22 @@ __global__ void __launch_bounds__(BS) k_coarse_solve(CoarseView cv, const TI *in
 
 // u += P e_c, one thread per (node, rhs). Block 0 also writes r_c . e_c
 // per rhs as one partial set in the (gamma, rr) layout of the PCG's update
-// partials (rr slot 0), summed in row order.
+// partials (rr slot 0), summed in a fixed order (block_sum_by_rhs).
 template <typename T, int ND, int NRP>
 __global__ void __launch_bounds__(256) k_coarse_prolong(tfm_mesh_t m, CoarseView cv,
                                                         const uint8_t *con, T *u, double *part,
                                                         const int *alldone) {
   constexpr int NM = coarse_modes<ND>();
-  __shared__ double sh[256];
+  static_assert(BS == 256, "k_coarse_prolong runs blocks of BS = 256 threads");
+  __shared__ double sh[BS / 64 * NRP];
   if (*alldone) return;
   if (blockIdx.x == 0) {
     const int rr = threadIdx.x % NRP, c = threadIdx.x / NRP;
     double d = 0.0;
-    for (int row = c; row < cv.n_c; row += 256 / NRP)
+    for (int row = c; row < cv.n_c; row += BS / NRP)
       d += cv.rc[(int64_t)row * NRP + rr] * cv.ec[(int64_t)row * NRP + rr];
-    sh[threadIdx.x] = d;
-    __syncthreads();
+    const double t2 = block_sum_by_rhs<NRP>(d, sh);
     if (threadIdx.x < NRP) {
-      double t2 = 0.0;
-      for (int cc = 0; cc < 256 / NRP; ++cc) t2 += sh[cc * NRP + threadIdx.x];
       part[threadIdx.x] = t2;
       part[NRP + threadIdx.x] = 0.0;
     }
@@ -1508,6 +1519,159 @@ __global__ void __launch_bounds__(256) k_coarse_prolong(tfm_mesh_t m, CoarseView
     for (int k = 0; k < NM; ++k) v += M[a][k] * e[k];
     T *ua = u + (i * ND + a) * NRP + rr;
     *ua = (T)((double)*ua + v);
+  }
+}
+
+// ====================================================================
+// Dense SPD inverse by block Gauss-Jordan (the coarse inverse of the
+// two-level preconditioner). With the pivot block K = [k0, k0 + nb) and
+// D = A_KK^{-1}, one step maps
+//   A_KK -> D,   A_iK -> -A_iK D,   A_Kj -> D A_Kj,
+//   A_ij -> A_ij - A_iK D A_Kj                      (i, j outside K),
+// and after a step over every block the matrix holds A^{-1}. Each pivot
+// block is then a Schur complement of an SPD matrix, so no pivoting is
+// needed; a pivot that is not positive sets the status. A step is three
+// launches: the pivot block's inverse (one block, in LDS), the panel
+// R = [-A_iK D; D] and the rank-nb update into the other buffer.
+// ====================================================================
+constexpr int GJ_NB = 64;  // pivot block
+constexpr int GJ_T = 64;   // update tile, rows and columns
+constexpr int GJ_QC = 32;  // rows of the pivot panel staged at once in the update
+constexpr int GJ_PR = 16;  // rows per block of the panel kernel
+
+// D [GJ_NB x GJ_NB] = A_KK^{-1} by scalar Gauss-Jordan, the block held in
+// registers: thread t owns row t / 4, columns 16 (t % 4) .. 16 (t % 4) + 15.
+// The pivot's row and column go through LDS, double-buffered, so a step has
+// one barrier. A block narrower than GJ_NB is padded with the identity,
+// whose inverse it keeps.
+__global__ void __launch_bounds__(256) k_gj_pivot(const double *a, int64_t n, int k0, int nb,
+                                                  double *D, int32_t *status) {
+  __shared__ double colb[2][GJ_NB], rowb[2][GJ_NB];
+  const int t = threadIdx.x;
+  const int i = t / 4, j0 = (t % 4) * 16;
+  double v[16];
+  for (int jj = 0; jj < 16; ++jj) {
+    const int j = j0 + jj;
+    v[jj] = (i < nb && j < nb) ? a[(int64_t)(k0 + i) * n + k0 + j] : (i == j ? 1.0 : 0.0);
+  }
+  int bad = -1;
+  for (int pb = 0; pb < GJ_NB; pb += 16) {
+#pragma unroll
+    for (int pp = 0; pp < 16; ++pp) {
+      const int p = pb + pp, buf = pp & 1;
+      if (j0 == pb) colb[buf][i] = v[pp];
+      if (i == p)
+        for (int jj = 0; jj < 16; ++jj) rowb[buf][j0 + jj] = v[jj];
+      __syncthreads();
+      double piv = rowb[buf][p];
+      if (!(piv > 0.0)) {
+        if (bad < 0) bad = p;
+        piv = 1.0;
+      }
+      const double inv = 1.0 / piv;
+      const double ci = colb[buf][i] * inv;
+      for (int jj = 0; jj < 16; ++jj) {
+        const int j = j0 + jj;
+        const double rj = rowb[buf][j];
+        double w;
+        if (i == p)
+          w = j == p ? inv : inv * rj;
+        else if (j == p)
+          w = -ci;
+        else
+          w = v[jj] - ci * rj;
+        v[jj] = w;
+      }
+    }
+  }
+  for (int jj = 0; jj < 16; ++jj) D[i * GJ_NB + j0 + jj] = v[jj];
+  if (t == 0 && bad >= 0 && *status == 0) *status = k0 + bad + 1;
+}
+
+// Rt [nb x n] = R^T with R[i][q] = D[i - k0][q] for i in K, else
+// -sum_p A[i][k0 + p] D[p][q]; GJ_PR rows per block, four columns q per
+// thread.
+__global__ void __launch_bounds__(256) k_gj_panel(const double *a, int64_t n, int k0, int nb,
+                                                  const double *D, double *Rt) {
+  __shared__ double Ds[GJ_NB][GJ_NB];
+  __shared__ double As[GJ_PR][GJ_NB + 1];  // As[r][p] = A[i0 + r][k0 + p]
+  const int t = threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.x * GJ_PR;
+  for (int e = t; e < GJ_NB * GJ_NB; e += 256) Ds[e / GJ_NB][e % GJ_NB] = D[e];
+  for (int e = t; e < GJ_PR * GJ_NB; e += 256) {
+    const int r = e / GJ_NB, q = e % GJ_NB;
+    const int64_t i = i0 + r;
+    As[r][q] = (i < n && q < nb) ? a[i * n + k0 + q] : 0.0;
+  }
+  __syncthreads();
+  const int r = t % GJ_PR, qg = (t / GJ_PR) * 4;
+  const int64_t i = i0 + r;
+  if (i >= n) return;
+  const bool inK = i >= k0 && i < k0 + nb;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (!inK)
+    for (int p = 0; p < GJ_NB; ++p) {
+      const double x = As[r][p];
+      for (int k = 0; k < 4; ++k) acc[k] += x * Ds[p][qg + k];
+    }
+  for (int k = 0; k < 4; ++k) {
+    const int q = qg + k;
+    if (q >= nb) break;
+    Rt[(int64_t)q * n + i] = inK ? Ds[i - k0][q] : -acc[k];
+  }
+}
+
+// dst = the step's image of src, one 64 x 64 tile per block:
+//   j in K:      dst[i][j] = R[i][j - k0]
+//   j outside K: dst[i][j] = (i in K ? 0 : src[i][j]) + sum_q R[i][q] src[k0 + q][j].
+// Thread (ty, tx) owns rows 4 ty .. 4 ty + 3 and columns 4 tx .. 4 tx + 3;
+// its src values are loaded before the product so that their latency
+// overlaps the staging.
+__global__ void __launch_bounds__(256) k_gj_update(const double *src, double *dst, int64_t n,
+                                                   int k0, int nb, const double *Rt) {
+  __shared__ double Rs[GJ_QC][GJ_T];  // Rs[q][r] = R[i0 + r][q0 + q], zero beyond nb
+  __shared__ double Bs[GJ_QC][GJ_T];  // Bs[q][c] = src[k0 + q0 + q][j0 + c]
+  const int t = threadIdx.x, tx = t % 16, ty = t / 16;
+  const int64_t i0 = (int64_t)blockIdx.y * GJ_T, j0 = (int64_t)blockIdx.x * GJ_T;
+  double acc[4][4], base[4][4];
+  for (int x = 0; x < 4; ++x) {
+    const int64_t i = i0 + ty * 4 + x;
+    const bool iK = i >= k0 && i < k0 + nb;
+    for (int y = 0; y < 4; ++y) {
+      const int64_t j = j0 + tx * 4 + y;
+      const bool jK = j >= k0 && j < k0 + nb;
+      base[x][y] = (i < n && j < n && !iK && !jK) ? src[i * n + j] : 0.0;
+      acc[x][y] = 0.0;
+    }
+  }
+  for (int q0 = 0; q0 < nb; q0 += GJ_QC) {
+    const int qn = min(GJ_QC, nb - q0);
+    for (int e = t; e < GJ_QC * GJ_T; e += 256) {
+      const int q = e / GJ_T, c = e % GJ_T;
+      const bool okq = q < qn;
+      const int64_t i = i0 + c, j = j0 + c;
+      Rs[q][c] = (okq && i < n) ? Rt[(int64_t)(q0 + q) * n + i] : 0.0;
+      Bs[q][c] = (okq && j < n) ? src[(int64_t)(k0 + q0 + q) * n + j] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int q = 0; q < GJ_QC; ++q) {
+      double rv[4], bv[4];
+      for (int x = 0; x < 4; ++x) rv[x] = Rs[q][ty * 4 + x];
+      for (int y = 0; y < 4; ++y) bv[y] = Bs[q][tx * 4 + y];
+      for (int x = 0; x < 4; ++x)
+        for (int y = 0; y < 4; ++y) acc[x][y] += rv[x] * bv[y];
+    }
+    __syncthreads();
+  }
+  for (int x = 0; x < 4; ++x) {
+    const int64_t i = i0 + ty * 4 + x;
+    if (i >= n) break;
+    for (int y = 0; y < 4; ++y) {
+      const int64_t j = j0 + tx * 4 + y;
+      if (j >= n) break;
+      dst[i * n + j] = (j >= k0 && j < k0 + nb) ? Rt[(j - k0) * n + i] : base[x][y] + acc[x][y];
+    }
   }
 }
 
@@ -1762,6 +1926,40 @@ extern "C" int tfm_coarse_set_inverse(tfm_coarse_t *h, const double *inv64, cons
 extern "C" int tfm_pcg_set_coarse(tfm_pcg_t *p, tfm_coarse_t *c) {
   TFM_REQUIRE(p != nullptr, "null solver handle");
   p->coarse = c;
+  return TFM_OK;
+}
+
+extern "C" int tfm_spd_inverse_work(int32_t n, int64_t *count) {
+  TFM_REQUIRE(count != nullptr, "null argument");
+  TFM_REQUIRE(n >= 1 && n <= 32768, "matrix size out of range (1 .. 32768)");
+  *count = (int64_t)n * n + (int64_t)GJ_NB * n + GJ_NB * GJ_NB;
+  return TFM_OK;
+}
+
+extern "C" int tfm_spd_inverse(tfm_stream_t stream, int32_t n, double *a, double *work,
+                               int32_t *status) {
+  TFM_REQUIRE(a && work && status, "null argument");
+  TFM_REQUIRE(n >= 1 && n <= 32768, "matrix size out of range (1 .. 32768)");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t nn = (int64_t)n * n;
+  double *buf[2] = {a, work};
+  double *Rt = work + nn;
+  double *D = Rt + (int64_t)GJ_NB * n;
+  const unsigned tiles = (unsigned)((n + GJ_T - 1) / GJ_T);
+  int cur = 0;
+  for (int k0 = 0; k0 < n; k0 += GJ_NB) {
+    const int nb = std::min(GJ_NB, n - k0);
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, s, buf[cur], (int64_t)n, k0, nb, D,
+                       status);
+    hipLaunchKernelGGL(k_gj_panel, dim3((unsigned)((n + GJ_PR - 1) / GJ_PR)), dim3(256), 0, s,
+                       buf[cur], (int64_t)n, k0, nb, D, Rt);
+    hipLaunchKernelGGL(k_gj_update, dim3(tiles, tiles), dim3(256), 0, s, buf[cur], buf[cur ^ 1],
+                       (int64_t)n, k0, nb, Rt);
+    TFM_CHECK_LAUNCH();
+    cur ^= 1;
+  }
+  if (cur == 1)
+    TFM_CHECK_HIP(hipMemcpyAsync(a, work, sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
   return TFM_OK;
 }
 

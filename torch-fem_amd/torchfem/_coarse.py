@@ -10,7 +10,7 @@ for about one coarse unknown per 128 fine ones and at most COARSE_SIZE
 1,536 with the constant for a scalar field). The index work below runs once
 per pattern with torch on the device. The coarse operator P^T K P is formed
 per system by a HIP kernel (tfm_coarse_operator), and its dense inverse by
-torch.linalg (Cholesky) on the device.
+the library's block Gauss-Jordan (tfm_spd_inverse) on the device.
 """
 
 from __future__ import annotations
@@ -25,6 +25,20 @@ from torch import Tensor
 from . import _lib as L
 
 COARSE_SIZE = 1536  # most coarse unknowns (timed on the 64^3 cube: 768, 1536, 3072)
+
+
+def spd_inverse(A: Tensor, stream=None) -> Tensor | None:
+    """A^{-1} of a dense SPD FP64 matrix on the device, in place
+    (tfm_spd_inverse: block Gauss-Jordan); None if a pivot is not positive."""
+    n = A.shape[0]
+    count = L.I64()
+    L.check(L.lib().tfm_spd_inverse_work(n, C.byref(count)))
+    work = torch.empty(int(count.value), dtype=torch.float64, device=A.device)
+    status = torch.zeros(1, dtype=torch.int32, device=A.device)
+    if stream is None:
+        stream = L.stream_of(A.device)
+    L.check(L.lib().tfm_spd_inverse(stream, n, L.ptr(A), L.ptr(work), L.ptr(status)))
+    return A if int(status) == 0 else None
 
 
 class CoarseSpace:
@@ -66,8 +80,11 @@ class CoarseSpace:
         counts = torch.bincount(agg, minlength=n_agg)
         agg_ptr = torch.zeros(n_agg + 1, dtype=torch.int64, device=dev)
         agg_ptr[1:] = torch.cumsum(counts, 0)
-        cen = torch.zeros(n_agg, D, dtype=torch.float64, device=dev).index_add_(0, agg, X)
-        cen = cen / counts[:, None].double()
+        # centroids: segment sums over the aggregate-sorted nodes (an ordered
+        # scan, repeatable bit for bit, where index_add_ adds with atomics)
+        csum = torch.cat([torch.zeros(D, 1, dtype=torch.float64, device=dev),
+                          torch.cumsum(X[agg_nodes].double().T.contiguous(), 1)], 1)
+        cen = ((csum[:, agg_ptr[1:]] - csum[:, agg_ptr[:-1]]) / counts.double()).T.contiguous()
         acell = torch.stack([uniq // (nc[1] * nc[2]), (uniq // nc[2]) % nc[1], uniq % nc[2]], 1)
         # cell -> aggregate, then the aggregate of each of the 27 slots
         cmap = torch.full((nc[0] * nc[1] * nc[2],), -1, dtype=torch.int64, device=dev)
@@ -126,8 +143,9 @@ class CoarseSpace:
         scale); coarse modes that vanish (aggregates whose DOFs are all
         constrained) get a unit diagonal, and a 1e-12 shift keeps the factor
         defined when an aggregate's modes are dependent (their prolongation is
-        zero either way). If the Cholesky factorization still fails, the
-        eigenvalues below 1e-12 of the largest are dropped."""
+        zero either way). The inverse is the library's block Gauss-Jordan
+        (tfm_spd_inverse); if one of its pivots is not positive, the
+        eigenvalues below 1e-12 of the largest are dropped instead."""
         pat = self.pat
         Ac = torch.empty(self.n_c, self.n_c, dtype=torch.float64, device=pat.device)
         L.check(L.lib().tfm_coarse_operator(pat.stream(), self.h, C.byref(system.sys), L.ptr(Ac)))
@@ -140,13 +158,11 @@ class CoarseSpace:
         sc = 1.0 / torch.sqrt(dg)
         A = sc[:, None] * Ac * sc[None, :]
         A.diagonal().copy_(torch.where(dead, torch.ones_like(dg), A.diagonal() + 1e-12))
-        Lc, info = torch.linalg.cholesky_ex(A)
-        if int(info) != 0:
+        inv = spd_inverse(A.clone(), pat.stream())
+        if inv is None:
             w, V = torch.linalg.eigh(A)
             w = torch.where(w > 1e-12 * float(w.abs().max()), 1.0 / w, torch.zeros_like(w))
             inv = (V * w) @ V.T
-        else:
-            inv = torch.cholesky_inverse(Lc)
         inv = sc[:, None] * (0.5 * (inv + inv.T)) * sc[None, :]
         inv = inv.contiguous()
         return inv, inv.float().contiguous()

@@ -154,6 +154,8 @@ _SIGNATURES: dict[str, tuple[list[Any], Any]] = {
     "tfm_coarse_size": ([P, C.POINTER(I32)], C.c_int),
     "tfm_coarse_operator": ([P, P, P, P], C.c_int),
     "tfm_coarse_set_inverse": ([P, P, P], C.c_int),
+    "tfm_spd_inverse_work": ([I32, C.POINTER(I64)], C.c_int),
+    "tfm_spd_inverse": ([P, I32, P, P, P], C.c_int),
     "tfm_pcg_set_coarse": ([P, P], C.c_int),
     "tfm_csr_pcg_solve": (
         [P, P, I64, P, P, P, P, P, F64, C.POINTER(I32), C.POINTER(F64)],

@@ -1404,9 +1404,9 @@ __global__ void __launch_bounds__(64) k_coarse_operator(tfm_mesh_t m, CoarseView
 
 // Sums of v over the lanes of each wave that share lane % NRP (a butterfly
 // in a fixed order), then over the block's waves in wave order, for the
-// values of one block of BS threads; the result is valid in threads
+// values of one block of NT threads; the result is valid in threads
 // 0 .. NRP - 1 (rhs = thread). Every thread of the block must call it.
-template <int NRP>
+template <int NRP, int NT = BS>
 __device__ __forceinline__ double block_sum_by_rhs(double v, double *sh) {
   for (int off = 32; off >= NRP; off >>= 1) v += __shfl_xor(v, off, 64);
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
@@ -1414,14 +1414,15 @@ __device__ __forceinline__ double block_sum_by_rhs(double v, double *sh) {
   __syncthreads();
   double tsum = 0.0;
   if (threadIdx.x < NRP)
-    for (int w = 0; w < BS / 64; ++w) tsum += sh[w * NRP + threadIdx.x];
+    for (int w = 0; w < NT / 64; ++w) tsum += sh[w * NRP + threadIdx.x];
   __syncthreads();
   return tsum;
 }
 
 // r_c = P^T r: one block per aggregate; thread t takes rhs t % NRP of every
 // (BS / NRP)-th node of the aggregate, and the block adds the thread sums by
-// waves (block_sum_by_rhs).
+// waves (block_sum_by_rhs). (Blocks of 1024 threads, one node per thread,
+// measured slower: 11.2 against 8.9 us on the 64^3 cube.)
 template <typename T, int ND, int NRP>
 __global__ void __launch_bounds__(BS) k_coarse_restrict(tfm_mesh_t m, CoarseView cv,
                                                         const uint8_t *con, const T *r,
@@ -1625,23 +1626,22 @@ __global__ void __launch_bounds__(256) k_gj_panel(const double *a, int64_t n, in
 //   j in K:      dst[i][j] = R[i][j - k0]
 //   j outside K: dst[i][j] = (i in K ? 0 : src[i][j]) + sum_q R[i][q] src[k0 + q][j].
 // Thread (ty, tx) owns rows 4 ty .. 4 ty + 3 and columns 4 tx .. 4 tx + 3;
-// its src values are loaded before the product so that their latency
-// overlaps the staging.
+// its accumulators start from its src values, loaded before the product so
+// that their latency overlaps the staging.
 __global__ void __launch_bounds__(256) k_gj_update(const double *src, double *dst, int64_t n,
                                                    int k0, int nb, const double *Rt) {
   __shared__ double Rs[GJ_QC][GJ_T];  // Rs[q][r] = R[i0 + r][q0 + q], zero beyond nb
   __shared__ double Bs[GJ_QC][GJ_T];  // Bs[q][c] = src[k0 + q0 + q][j0 + c]
   const int t = threadIdx.x, tx = t % 16, ty = t / 16;
   const int64_t i0 = (int64_t)blockIdx.y * GJ_T, j0 = (int64_t)blockIdx.x * GJ_T;
-  double acc[4][4], base[4][4];
+  double acc[4][4];  // starts from the src values
   for (int x = 0; x < 4; ++x) {
     const int64_t i = i0 + ty * 4 + x;
     const bool iK = i >= k0 && i < k0 + nb;
     for (int y = 0; y < 4; ++y) {
       const int64_t j = j0 + tx * 4 + y;
       const bool jK = j >= k0 && j < k0 + nb;
-      base[x][y] = (i < n && j < n && !iK && !jK) ? src[i * n + j] : 0.0;
-      acc[x][y] = 0.0;
+      acc[x][y] = (i < n && j < n && !iK && !jK) ? src[i * n + j] : 0.0;
     }
   }
   for (int q0 = 0; q0 < nb; q0 += GJ_QC) {
@@ -1670,7 +1670,7 @@ __global__ void __launch_bounds__(256) k_gj_update(const double *src, double *ds
     for (int y = 0; y < 4; ++y) {
       const int64_t j = j0 + tx * 4 + y;
       if (j >= n) break;
-      dst[i * n + j] = (j >= k0 && j < k0 + nb) ? Rt[(j - k0) * n + i] : base[x][y] + acc[x][y];
+      dst[i * n + j] = (j >= k0 && j < k0 + nb) ? Rt[(j - k0) * n + i] : acc[x][y];
     }
   }
 }

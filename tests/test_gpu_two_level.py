@@ -4,8 +4,10 @@ modes, in the place of the reference's AMG preconditioner,
 sparse.py:422-442). Results against the oracle's direct solves for every
 element type and both precisions, gradients against the SIMP closed form,
 batched load cases, aggregates whose DOFs are all constrained, the solver
-node order of Hexa2 meshes and the iteration counts it saves. Void density
-designs are covered with the other preconditioners in test_gpu_contrast.py."""
+node order of Hexa2 meshes, Newton increments and implicit time steps,
+the dense inverse of the coarse operator and the iteration counts it saves.
+Void density designs are covered with the other preconditioners in
+test_gpu_contrast.py."""
 
 import pytest
 import torch
@@ -184,6 +186,49 @@ def test_spd_inverse(n):
         B = torch.eye(n, dtype=DT, device="cuda")
         B[n - 1, n - 1] = -1.0
         assert spd_inverse(B) is None
+
+
+def test_newton_increments_and_time_stepping():
+    """Inside Newton increments (a Neo-Hookean cantilever with nlgeom, 2,499
+    DOFs in four aggregates: every tangent gets its own coarse operator and
+    inverse) and inside implicit time stepping (a 33 x 33 heated plate,
+    (M + dt/2 K) solves): the results equal block-Jacobi's."""
+    from test_gpu_hyper import LBD, MU, psi
+    from torchfem import PlanarHeat
+    from torchfem.materials import Hyperelastic3D, IsotropicConductivity2D
+    nodes, el = cube_hexa(17, 7, 7)
+    nodes = nodes * torch.tensor([2.0, 0.5, 0.5], dtype=DT)
+    params = torch.tensor([MU, LBD], dtype=DT)
+    out = {}
+    for pc in ("block_jacobi", "two_level"):
+        beam = Solid(nodes.cuda(), el.cuda(), Hyperelastic3D(psi, params))
+        X = beam.nodes
+        beam.constraints[X[:, 0] == 0.0] = True
+        tip = X[:, 0] == 2.0
+        beam.forces[tip, 2] = -0.36 / int(tip.sum())
+        beam.forces[tip, 1] = 0.108 / int(tip.sum())
+        u, f, _, _, _ = beam.solve(increments=torch.tensor([0.0, 0.5, 1.0]), nlgeom=True,
+                                   rtol=1e-12, atol=1e-9, stol=1e-12, max_iter=40,
+                                   precision="fp64", preconditioner=pc)
+        out[pc] = (u, f)
+    assert float(out["two_level"][0].abs().max()) > 1e-3
+    assert rel(out["two_level"][0], out["block_jacobi"][0]) < 1e-9
+    assert rel(out["two_level"][1], out["block_jacobi"][1]) < 1e-8
+
+    nodes, el = rect_quad(33, 33, 1.0, 1.0)
+    T = {}
+    for pc in ("block_jacobi", "two_level"):
+        plate = PlanarHeat(nodes.cuda(), el.cuda(), IsotropicConductivity2D(kappa=400.0, rho=1.0e5))
+        west = torch.isclose(plate.nodes[:, 0], plate.nodes[:, 0].min())
+        east = torch.isclose(plate.nodes[:, 0], plate.nodes[:, 0].max())
+        plate.constraints[west | east] = True
+        plate.temperatures[west, 0] = 5.0
+        plate.temperatures[east, 0] = 20.0
+        T[pc] = plate.time_integration(torch.tensor([0.0, 0.7, 2.0]), delta_t=0.25, rtol=1e-12,
+                                       atol=1e-12, stol=1e-12, precision="mixed",
+                                       preconditioner=pc)[0]
+    assert float((T["two_level"][-1] - 5.0).abs().max()) > 1.0
+    assert rel(T["two_level"], T["block_jacobi"]) < 1e-9
 
 
 def test_iterations_drop():

@@ -231,6 +231,33 @@ def test_newton_increments_and_time_stepping():
     assert rel(T["two_level"], T["block_jacobi"]) < 1e-9
 
 
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_every_element_case(precision):
+    """Every element type and material layout of the kernel tests (plane
+    stress with thickness, heat in 2-D and 3-D, Tetra1 / Hexa1 / Hexa2
+    elasticity, isotropic and general tangents), clamped at the lowest x and
+    loaded at the highest: the two-level solve against the oracle's direct
+    solve."""
+    from test_gpu_kernels import CASES, _case
+    for name, general in CASES:
+        m, o = _case(name, general)
+        X = m.nodes
+        m.constraints[X[:, 0] == X[:, 0].min()] = True
+        far = X[:, 0] == X[:, 0].max()
+        heat = name.endswith("heat")
+        if heat:
+            m.heat_flux[far] = 1.0 / int(far.sum())
+            load, pres = m.heat_flux, m.temperatures
+        else:
+            m.forces[far, -1] = -1.0 / int(far.sum())
+            m.forces[far, 0] = 0.4 / int(far.sum())
+            load, pres = m.forces, m.displacements
+        u, f, _, _, _ = m.solve(precision=precision, preconditioner="two_level", stol=1e-11)
+        uo, fo, _, _ = o.solve(m.constraints.cpu(), load.cpu(), pres.cpu())
+        assert rel(u, uo) < 1e-9, (name, general)
+        assert rel(f, fo) < 1e-8, (name, general)
+
+
 def test_coarse_inverse_fallback(monkeypatch):
     """When the Gauss-Jordan inverse reports a pivot that is not positive,
     the coarse inverse comes from the eigendecomposition instead (forced

@@ -258,6 +258,64 @@ def test_every_element_case(precision):
         assert rel(f, fo) < 1e-8, (name, general)
 
 
+@pytest.mark.parametrize("kind", ["solid", "planar", "heat"])
+def test_coarse_operator_on_a_free_body(kind):
+    """k_coarse_operator (A_c = P^T K P) on an unconstrained body, whose K
+    has the rigid motions (the constants for heat) as its null space: A_c is
+    symmetric, annihilates the coefficients of every global rigid motion
+    (translation t + rotation w about the origin is t + w x c_I plus the
+    rotation w in the modes of aggregate I, whose centroid is c_I), and has
+    exactly that many zero eigenvalues (6, 3 or 1). A rotation mode with a
+    wrong sign or axis, or a dropped block, fails one of these."""
+    import ctypes as C
+
+    from torchfem import _lib as L
+    from torchfem._coarse import CoarseSpace
+    from torchfem.base import _System
+    from torchfem.materials import IsotropicConductivity3D
+    if kind == "solid":
+        nodes, el, E = _hexa(7)
+        m = Solid(nodes.cuda(), el.cuda(), IsotropicElasticity3D(E.cuda(), 0.3))
+        nz = 6
+    elif kind == "planar":
+        nodes, el = O.rect_quad(25, 9, 3.0, 1.0)
+        m = Planar(nodes.cuda(), el.cuda(), IsotropicElasticityPlaneStress(1000.0, 0.3))
+        nz = 3
+    else:
+        nodes, el = O.cube_tetra6(6, jitter=0.1, seed=2)
+        m = SolidHeat(nodes.cuda(), el.cuda(), IsotropicConductivity3D(1.0))
+        nz = 1
+    kd, p0, p1, sc = m._material_params()
+    S = _System(m, kd, p0, p1, sc, "fp64", renumber=False)
+    pat = S._pat
+    cs = CoarseSpace(pat, size=24 * cs_modes(kind))
+    assert cs.n_agg >= 4
+    Ac = torch.empty(cs.n_c, cs.n_c, dtype=DT, device="cuda")
+    L.check(L.lib().tfm_coarse_operator(pat.stream(), cs.h, C.byref(S.sys), L.ptr(Ac)))
+    Ac = Ac.cpu()
+    scale = float(Ac.abs().max())
+    assert float((Ac - Ac.T).abs().max()) < 1e-12 * scale
+    cen = cs._arrays["cen"].cpu()
+    g = torch.Generator().manual_seed(5)
+    if kind == "heat":
+        c = torch.ones(cs.n_c, dtype=DT)
+    elif kind == "solid":
+        t, w = torch.randn(3, generator=g, dtype=DT), torch.randn(3, generator=g, dtype=DT)
+        c = torch.cat([t + torch.linalg.cross(w.expand_as(cen), cen), w.expand_as(cen)], 1).reshape(-1)
+    else:
+        t, w = torch.randn(2, generator=g, dtype=DT), float(torch.randn(1, generator=g, dtype=DT))
+        rot = torch.stack([-w * cen[:, 1], w * cen[:, 0]], 1)
+        c = torch.cat([t + rot, torch.full((cs.n_agg, 1), w, dtype=DT)], 1).reshape(-1)
+    assert float((Ac @ c).norm()) < 1e-10 * scale * float(c.norm())
+    ev = torch.linalg.eigvalsh(Ac)
+    assert bool((ev[:nz].abs() < 1e-9 * scale).all())
+    assert float(ev[nz]) > 1e-6 * scale
+
+
+def cs_modes(kind):
+    return {"solid": 6, "planar": 3, "heat": 1}[kind]
+
+
 def test_coarse_inverse_fallback(monkeypatch):
     """When the Gauss-Jordan inverse reports a pivot that is not positive,
     the coarse inverse comes from the eigendecomposition instead (forced

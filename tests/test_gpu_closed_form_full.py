@@ -32,6 +32,17 @@ from torchfem.mesh import cube_hexa, cube_tetra6
 
 pytestmark = pytest.mark.gpu
 DT = torch.float64
+
+
+@pytest.fixture(autouse=True, params=["block_jacobi", "two_level"])
+def preconditioner(request):
+    """Every full-size case runs with both preconditioners of the FEM solves
+    (the default, "auto", would pick the two-level one at these sizes)."""
+    from torchfem.sparse import default_preconditioner, set_default_preconditioner
+    old = default_preconditioner()
+    set_default_preconditioner(request.param)
+    yield request.param
+    set_default_preconditioner(old)
 CASES = [("fp64", 1e-12, 1e-10), ("mixed", 1e-10, 1e-8)]
 
 G_LIN = torch.tensor([[1.0e-3, 2.0e-4, -1.0e-4], [3.0e-4, -1.0e-3, 5.0e-5],

@@ -17,6 +17,17 @@ pytestmark = pytest.mark.gpu
 DT = torch.float64
 
 
+@pytest.fixture(autouse=True, params=["block_jacobi", "two_level"])
+def preconditioner(request):
+    """Every full-size case runs with both preconditioners of the FEM solves
+    (the default, "auto", would pick the two-level one at these sizes)."""
+    from torchfem.sparse import default_preconditioner, set_default_preconditioner
+    old = default_preconditioner()
+    set_default_preconditioner(request.param)
+    yield request.param
+    set_default_preconditioner(old)
+
+
 def rel(a, b):
     a = a.detach().double()
     b = b.detach().double()

@@ -45,7 +45,8 @@ from ._coarse import CoarseSpace
 INNER_RTOL = 0.1
 INNER_RTOL_TWO_LEVEL = 0.03
 from ._finite import Kinematics, solve_finite
-from .sparse import CachedSolve, PCGSolver, default_precision
+from .sparse import (AUTO_TWO_LEVEL_DOFS, PRECONDITIONERS, CachedSolve, PCGSolver,
+                     default_precision, default_preconditioner)
 
 _PREC = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED}
 _PC = {"jacobi": L.PC_JACOBI, "block_jacobi": L.PC_BLOCK_JACOBI, "two_level": L.PC_TWO_LEVEL}
@@ -495,6 +496,17 @@ class FEM(ABC):
             return out
         return x
 
+    def _resolve_pc(self, preconditioner: str | None) -> str:
+        """The preconditioner a solve runs with: None takes
+        default_preconditioner(); "auto" the two-level preconditioner from
+        AUTO_TWO_LEVEL_DOFS unknowns, block-Jacobi below."""
+        name = preconditioner or default_preconditioner()
+        if name not in PRECONDITIONERS:
+            raise ValueError(f"preconditioner must be one of {sorted(PRECONDITIONERS)}")
+        if name == "auto":
+            name = "two_level" if self.n_dofs >= AUTO_TWO_LEVEL_DOFS else "block_jacobi"
+        return name
+
     def _coarse_for(self, system):
         """The two-level preconditioner's coarse space for the system's
         pattern (built once per pattern), holding the inverse of the system's
@@ -821,20 +833,25 @@ class FEM(ABC):
         alpha: float = 0.0,
         differentiable_parameters: Tensor | Iterable[Tensor] | None = None,
         precision: str | None = None,
-        preconditioner: str = "block_jacobi",
+        preconditioner: str | None = None,
         inner_rtol: float | None = None,
     ):
         """Quasi-static solve by load increments (base.py:597-923).
 
-        Extra keywords of this build: `precision` ("mixed": FP32-inner PCG with
-        FP64 refinement, the default; "fp64": all-FP64 PCG), `preconditioner`
-        ("block_jacobi", "jacobi", or "two_level": block-Jacobi with an additive
-        coarse correction on node aggregates and their rigid-body modes, in the
-        place of the reference's AMG) and `inner_rtol` (mixed precision: the
-        factor by which the FP32 recursive residual drops between two reliable
-        updates; None takes 0.1 with block-Jacobi or Jacobi and 0.03 with the
-        two-level preconditioner, whose solves take few iterations, so that
-        the fixed cost of each refinement weighs more).
+        Extra keywords of this build:
+        * `precision`: "mixed" (FP32-inner PCG with FP64 refinement, the
+          default) or "fp64" (all-FP64 PCG).
+        * `preconditioner`: "block_jacobi", "jacobi", "two_level"
+          (block-Jacobi with an additive coarse correction on node
+          aggregates and their rigid-body modes, in the place of the
+          reference's AMG) or "auto" (two-level from 100,000 unknowns,
+          block-Jacobi below). None takes default_preconditioner(), "auto"
+          unless set_default_preconditioner changed it.
+        * `inner_rtol` (mixed precision): the factor by which the FP32
+          recursive residual drops between two reliable updates. None takes
+          0.1 with block-Jacobi or Jacobi and 0.03 with the two-level
+          preconditioner, whose solves take few iterations, so that the fixed
+          cost of each refinement weighs more.
         """
         finite = getattr(self.material, "incremental", False)
         if nlgeom and not finite:
@@ -860,8 +877,7 @@ class FEM(ABC):
         precision = precision or default_precision()
         if precision not in _PREC:
             raise ValueError(f"precision must be one of {sorted(_PREC)}")
-        if preconditioner not in _PC:
-            raise ValueError(f"preconditioner must be one of {sorted(_PC)}")
+        preconditioner = self._resolve_pc(preconditioner)
 
         increments = torch.tensor([0.0, 1.0]) if increments is None else torch.as_tensor(increments)
         incs = [float(v) for v in increments]
@@ -1000,7 +1016,7 @@ class FEM(ABC):
         use_cached_solve: bool = False,
         differentiable_parameters: Tensor | Iterable[Tensor] | None = None,
         precision: str | None = None,
-        preconditioner: str = "block_jacobi",
+        preconditioner: str | None = None,
         inner_rtol: float | None = None,
     ):
         """Implicit time integration from an equilibrium state
@@ -1030,6 +1046,7 @@ class FEM(ABC):
         if getattr(self.material, "incremental", False) or self.n_state != 0:
             raise NotImplementedError("time integration supports linear materials in this build")
         precision = precision or default_precision()
+        preconditioner = self._resolve_pc(preconditioner)
         cfg = dict(precision=precision, pc=preconditioner, stol=stol, inner_rtol=inner_rtol)
         dev, dt64 = self.device, torch.float64
         nd = self.n_dof_per_node
@@ -1128,7 +1145,7 @@ class FEM(ABC):
         max_iter: int = 10,
         differentiable_parameters: Tensor | Iterable[Tensor] | None = None,
         precision: str | None = None,
-        preconditioner: str = "block_jacobi",
+        preconditioner: str | None = None,
         inner_rtol: float | None = None,
     ) -> tuple[Tensor, Tensor]:
         """Solves several load cases of one model at once (no counterpart in the
@@ -1152,6 +1169,7 @@ class FEM(ABC):
         else:
             params = tuple(differentiable_parameters)
         track = any(p.requires_grad for p in params)
+        preconditioner = self._resolve_pc(preconditioner)
         kind, p0, p1, scale = self._material_params()
         cfg = dict(kind=kind, precision=precision, pc=preconditioner, stol=stol, rtol=rtol,
                    atol=atol, max_iter=max_iter, inner_rtol=inner_rtol)

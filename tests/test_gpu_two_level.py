@@ -330,6 +330,22 @@ def test_coarse_inverse_fallback(monkeypatch):
     assert rel(f, fo) < 1e-8
 
 
+def test_coarse_space_failure_falls_back(monkeypatch):
+    """A coarse space that cannot be built (forced on the host by a stand-in
+    that raises) leaves the solve to block-Jacobi, marked in the solve info,
+    with the oracle's result."""
+    import torchfem.base as base
+
+    def refuse(pat, size=None):
+        raise RuntimeError("no coarse space")
+    monkeypatch.setattr(base, "CoarseSpace", refuse)
+    m, o = _solid_case(6)
+    u, f, _, _, _ = m.solve(precision="mixed", preconditioner="two_level", stol=1e-11)
+    assert m.last_solve_info["forward"]["pcg"][0].get("two_level_fallback")
+    uo, fo, _, _ = o.solve(m.constraints.cpu(), m.forces.cpu(), m.displacements.cpu())
+    assert rel(u, uo) < 1e-9
+
+
 def test_iterations_drop():
     """24^3 Hexa1 compression (the bench's load on a smaller cube): the
     coarse correction at least halves the PCG iterations of block-Jacobi,

@@ -460,7 +460,6 @@ class FEM(ABC):
         if perm is not None:  # to the solver's node order
             b, x = b[:, perm].contiguous(), x[:, perm].contiguous()
         solver = self._solver()
-        coarse = self._coarse_for(system) if cfg["pc"] == "two_level" else None
         # once the FP32 iterations of a mixed solve broke down on this system
         # (its FP32 operator is not positive definite), later solves with the
         # same matrix (the adjoint, further Newton steps) run in FP64 directly
@@ -468,17 +467,18 @@ class FEM(ABC):
         kw = dict(precision=precision, rtol=cfg["stol"],
                   final_rtol=(0.7 if adjoint else 0.25) * cfg["stol"])
         ir = cfg["inner_rtol"]
-        use_two = coarse is not None and not getattr(system, "coarse_failed", False)
+        use_two = cfg["pc"] == "two_level" and not getattr(system, "coarse_failed", False)
         if use_two:
             try:
+                coarse = self._coarse_for(system)
                 info = solver.solve_system(system.sys, b, x, preconditioner="two_level",
                                            coarse=coarse, inner_rtol=INNER_RTOL_TWO_LEVEL
                                            if ir is None else ir, **kw)
             except RuntimeError:
-                # the coarse correction did not reach the tolerance (seen on
-                # density designs with a 1e9 stiffness contrast): block-Jacobi
-                # continues from the iterate, here and for later solves with
-                # this matrix
+                # the coarse space could not be built, or the coarse correction
+                # did not reach the tolerance (seen on density designs with a
+                # 1e9 stiffness contrast): block-Jacobi continues from the
+                # iterate, here and for later solves with this matrix
                 system.coarse_failed = True
                 use_two = False
         if not use_two:

@@ -102,3 +102,32 @@ def test_cells_are_wider_than_elements():
     a = aggregates(X.to(DT), el, ptr, adj, X.shape[0] * 2, 2, size=10**6)
     # cells of at least 1.01 h: at most 10 along x for 10 elements
     assert int(a["agg_ptr"].numel()) - 1 <= 10
+
+
+def test_preconditioner_resolution():
+    """preconditioner=None takes the default, "auto": the two-level
+    preconditioner from AUTO_TWO_LEVEL_DOFS unknowns, block-Jacobi below;
+    explicit names pass through; set_default_preconditioner changes the
+    default; unknown names raise ValueError."""
+    from types import SimpleNamespace
+
+    from torchfem import sparse
+    from torchfem.base import FEM
+
+    small = SimpleNamespace(n_dofs=sparse.AUTO_TWO_LEVEL_DOFS - 1)
+    big = SimpleNamespace(n_dofs=sparse.AUTO_TWO_LEVEL_DOFS)
+    assert sparse.default_preconditioner() == "auto"
+    assert FEM._resolve_pc(small, None) == "block_jacobi"
+    assert FEM._resolve_pc(big, None) == "two_level"
+    assert FEM._resolve_pc(big, "auto") == "two_level"
+    assert FEM._resolve_pc(big, "jacobi") == "jacobi"
+    assert FEM._resolve_pc(small, "two_level") == "two_level"
+    with pytest.raises(ValueError):
+        FEM._resolve_pc(big, "amg")
+    sparse.set_default_preconditioner("block_jacobi")
+    try:
+        assert FEM._resolve_pc(big, None) == "block_jacobi"
+    finally:
+        sparse.set_default_preconditioner("auto")
+    with pytest.raises(ValueError):
+        sparse.set_default_preconditioner("multigrid")

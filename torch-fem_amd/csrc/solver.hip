@@ -1190,8 +1190,10 @@ static SpmvShape spmv_shape(const tfm_mesh_t &m, int nrp) {
   sh.scal = fp64;
   // FP64 with one or two rhs: the full grid (one node per team); with more,
   // a capped grid-stride grid, so that the one-block scalar kernel sums few
-  // partial sets (35k blocks x 16 rhs of Hexa2 32^3 took it 84 us)
-  sh.cap = fp64 ? (nrp >= 4 ? RBA / 2 : (1 << 30)) : (nrp <= 2 ? RBA : nrp <= 8 ? RBA / 2 : RBA / 4);
+  // partial sets (35k blocks x 16 rhs of Hexa2 32^3 took it 84 us). FP32
+  // with 16 rhs on Hexa2 32^3: 1024 blocks ran the step in 1,015 ms against
+  // 1,065 ms with 512
+  sh.cap = fp64 ? (nrp >= 4 ? RBA / 2 : (1 << 30)) : (nrp <= 2 ? RBA : RBA / 2);
   static const int g_env = env_int("TFM_SPMV_GRID", 0);
   static const int f_env = env_int("TFM_SPMV_SCAL", -1);
   if (g_env > 0) sh.cap = g_env;
